@@ -1,1 +1,11 @@
-"""fft_amd: MI355X-native engine behind the Signalsmith FFT API."""
+"""fft_amd: MI355X-native engine behind the Signalsmith FFT API.
+
+    FFT, RealFFT, ModifiedRealFFT   mirrors of signalsmith::FFT<V>, RealFFT<V>,
+                                    ModifiedRealFFT<V> (fft_amd/api.py)
+    Plan                            a cached engine plan (C ABI handle)
+    fill                            seeded synthetic inputs generated in HBM
+
+All transforms run through libsfft.so (include/sfft.h); importing this package
+does not load it -- the first call does, and raises if it is missing.
+"""
+from .api import FFT, ModifiedRealFFT, Plan, RealFFT, fill  # noqa: F401

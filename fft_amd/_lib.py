@@ -1,0 +1,111 @@
+"""ctypes binding of libsfft.so (the C ABI in include/sfft.h).
+
+The library is built in-tree by `make -C fft_amd/csrc` (or __graft_entry__.build())
+and loaded from fft_amd/libsfft.so.  There is no fallback: if the library is
+missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsfft.so")
+
+SFFT_OK = 0
+SFFT_ERR_INVALID_ARG = -1
+SFFT_ERR_UNSUPPORTED_SIZE = -2
+SFFT_ERR_OUT_OF_MEMORY = -3
+SFFT_ERR_DEVICE = -4
+SFFT_ERR_ALIASING = -5
+SFFT_ERR_NOT_READY = -6
+SFFT_ERR_BUDGET = -7
+
+C2C, R2C, R2C_HALFSHIFT = 0, 1, 2
+F32, F64 = 0, 1
+FORWARD, INVERSE = 0, 1
+
+# Every symbol include/sfft.h declares (checked by tests/test_capi_exports.py).
+EXPORTED = [
+    "sfft_init", "sfft_last_error", "sfft_device_info",
+    "sfft_size_minimum", "sfft_size_maximum", "sfft_real_size_minimum",
+    "sfft_real_size_maximum", "sfft_size_supported",
+    "sfft_plan_create", "sfft_plan_destroy", "sfft_plan_get_info",
+    "sfft_execute", "sfft_execute_strided",
+    "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_execute",
+    "sfft_batch_execute_per_size",
+    "sfft_fill",
+]
+
+
+class SfftError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sfft error {code}: {msg}")
+        self.code = code
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_size_t),
+        ("kind", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("passes", ctypes.c_int),
+        ("nradices", ctypes.c_int),
+        ("radices", ctypes.c_int * 16),
+        ("threads_per_transform", ctypes.c_int),
+        ("transforms_per_block", ctypes.c_int),
+        ("workspace_bytes", ctypes.c_size_t),
+        ("table_bytes", ctypes.c_size_t),
+    ]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_size_t),
+        ("kind", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("count", ctypes.c_size_t),
+        ("in_offset", ctypes.c_size_t),
+        ("out_offset", ctypes.c_size_t),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libsfft.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libsfft.so not built ({LIB_PATH}); run `make -C fft_amd/csrc` "
+                "or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        sz, vp, ip = ctypes.c_size_t, ctypes.c_void_p, ctypes.c_int
+        L.sfft_init.restype = ip
+        L.sfft_last_error.restype = ctypes.c_char_p
+        L.sfft_device_info.argtypes = [ctypes.POINTER(ip), ctypes.POINTER(sz), ctypes.c_char_p, sz]
+        for name in ("sfft_size_minimum", "sfft_size_maximum", "sfft_real_size_minimum",
+                     "sfft_real_size_maximum"):
+            getattr(L, name).argtypes = [sz]
+            getattr(L, name).restype = sz
+        L.sfft_size_supported.argtypes = [sz, ip]
+        L.sfft_plan_create.argtypes = [ctypes.POINTER(vp), sz, ip, ip]
+        L.sfft_plan_destroy.argtypes = [vp]
+        L.sfft_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
+        L.sfft_execute.argtypes = [vp, vp, vp, sz, ip, vp]
+        L.sfft_execute_strided.argtypes = [vp, vp, sz, vp, sz, sz, ip, vp]
+        L.sfft_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchItem), sz, ip]
+        L.sfft_batch_destroy.argtypes = [vp]
+        L.sfft_batch_execute.argtypes = [vp, vp, vp, vp]
+        L.sfft_batch_execute_per_size.argtypes = [vp, vp, vp, vp]
+        L.sfft_fill.argtypes = [vp, sz, ip, ip, ctypes.c_uint64, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != SFFT_OK:
+        msg = lib().sfft_last_error()
+        raise SfftError(rc, msg.decode() if msg else "")

@@ -1,0 +1,183 @@
+// capi.cpp -- the C ABI (include/sfft.h) over the engine.
+#include <sfft.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "batch.h"
+#include "engine.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const sfft::Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(SFFT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(SFFT_ERR_DEVICE, e.what());
+    }
+}
+
+}  // namespace
+
+struct sfft_plan_s {
+    std::shared_ptr<sfft::Plan> plan;
+};
+
+struct sfft_batch_s {
+    std::unique_ptr<sfft::UnevenBatch> impl;
+};
+
+extern "C" {
+
+int sfft_init(void) {
+    return guarded([] {
+        (void)sfft::device_info();
+        return SFFT_OK;
+    });
+}
+
+const char *sfft_last_error(void) { return g_last_error.c_str(); }
+
+int sfft_device_info(int *cu_count, size_t *lds_bytes, char *arch, size_t arch_len) {
+    return guarded([&] {
+        const sfft::DeviceInfo &d = sfft::device_info();
+        if (cu_count) *cu_count = d.cu_count;
+        if (lds_bytes) *lds_bytes = d.lds_bytes;
+        if (arch && arch_len) {
+            std::strncpy(arch, d.arch.c_str(), arch_len - 1);
+            arch[arch_len - 1] = 0;
+        }
+        return SFFT_OK;
+    });
+}
+
+size_t sfft_size_minimum(size_t n) { return sfft::size_minimum(n); }
+size_t sfft_size_maximum(size_t n) { return sfft::size_maximum(n); }
+size_t sfft_real_size_minimum(size_t n) { return sfft::real_size_minimum(n); }
+size_t sfft_real_size_maximum(size_t n) { return sfft::real_size_maximum(n); }
+
+int sfft_size_supported(size_t n, int kind) {
+    if (kind == SFFT_C2C) return sfft::supported_length(n) ? 1 : 0;
+    if (kind == SFFT_R2C || kind == SFFT_R2C_HALFSHIFT)
+        return (n >= 2 && n % 2 == 0 && sfft::supported_length(n / 2)) ? 1 : 0;
+    return 0;
+}
+
+int sfft_plan_create(sfft_plan *plan, size_t n, int kind, int precision) {
+    return guarded([&] {
+        if (!plan) return fail(SFFT_ERR_INVALID_ARG, "plan pointer is NULL");
+        *plan = nullptr;
+        auto p = sfft::get_plan(n, kind, precision);
+        *plan = new sfft_plan_s{p};
+        return SFFT_OK;
+    });
+}
+
+int sfft_plan_destroy(sfft_plan plan) {
+    delete plan;
+    return SFFT_OK;
+}
+
+int sfft_plan_get_info(sfft_plan plan, sfft_plan_info *info) {
+    return guarded([&] {
+        if (!plan || !info) return fail(SFFT_ERR_INVALID_ARG, "NULL argument");
+        const sfft::Plan &p = *plan->plan;
+        std::memset(info, 0, sizeof(*info));
+        info->n = p.n();
+        info->kind = p.kind();
+        info->precision = p.prec();
+        info->passes = (int)p.passes().size();
+        const sfft::PassDesc &pd = p.passes()[0].pd;
+        info->nradices = pd.nstages;
+        for (int s = 0; s < pd.nstages && s < 16; ++s) info->radices[s] = pd.st[s].radix;
+        info->threads_per_transform = pd.tpt;
+        info->transforms_per_block = pd.tpw;
+        info->workspace_bytes = 0;
+        info->table_bytes = p.table_bytes();
+        return SFFT_OK;
+    });
+}
+
+int sfft_execute(sfft_plan plan, const void *in, void *out, size_t batch, int direction,
+                 void *stream) {
+    return guarded([&] {
+        if (!plan) return fail(SFFT_ERR_INVALID_ARG, "plan is NULL");
+        if (direction != SFFT_FORWARD && direction != SFFT_INVERSE)
+            return fail(SFFT_ERR_INVALID_ARG, "direction must be SFFT_FORWARD or SFFT_INVERSE");
+        const sfft::Plan &p = *plan->plan;
+        p.execute(in, p.default_in_dist(direction), out, p.default_out_dist(direction), batch,
+                  direction, (hipStream_t)stream);
+        return SFFT_OK;
+    });
+}
+
+int sfft_execute_strided(sfft_plan plan, const void *in, size_t in_dist, void *out,
+                         size_t out_dist, size_t batch, int direction, void *stream) {
+    return guarded([&] {
+        if (!plan) return fail(SFFT_ERR_INVALID_ARG, "plan is NULL");
+        if (direction != SFFT_FORWARD && direction != SFFT_INVERSE)
+            return fail(SFFT_ERR_INVALID_ARG, "direction must be SFFT_FORWARD or SFFT_INVERSE");
+        plan->plan->execute(in, in_dist, out, out_dist, batch, direction, (hipStream_t)stream);
+        return SFFT_OK;
+    });
+}
+
+int sfft_batch_create(sfft_batch *batch, const sfft_batch_item *items, size_t count,
+                      int precision) {
+    return guarded([&] {
+        if (!batch || (!items && count)) return fail(SFFT_ERR_INVALID_ARG, "NULL argument");
+        *batch = nullptr;
+        auto impl = std::make_unique<sfft::UnevenBatch>(items, count, precision);
+        *batch = new sfft_batch_s{std::move(impl)};
+        return SFFT_OK;
+    });
+}
+
+int sfft_batch_destroy(sfft_batch batch) {
+    delete batch;
+    return SFFT_OK;
+}
+
+int sfft_batch_execute(sfft_batch batch, const void *in_arena, void *out_arena, void *stream) {
+    return guarded([&] {
+        if (!batch) return fail(SFFT_ERR_INVALID_ARG, "batch is NULL");
+        batch->impl->execute_persistent(in_arena, out_arena, (hipStream_t)stream);
+        return SFFT_OK;
+    });
+}
+
+int sfft_batch_execute_per_size(sfft_batch batch, const void *in_arena, void *out_arena,
+                                void *stream) {
+    return guarded([&] {
+        if (!batch) return fail(SFFT_ERR_INVALID_ARG, "batch is NULL");
+        batch->impl->execute_per_size(in_arena, out_arena, (hipStream_t)stream);
+        return SFFT_OK;
+    });
+}
+
+int sfft_fill(void *dst, size_t count, int precision, int dist, uint64_t seed, void *stream) {
+    return guarded([&] {
+        if (!dst && count) return fail(SFFT_ERR_INVALID_ARG, "NULL buffer");
+        if (dist != 0 && dist != 1) return fail(SFFT_ERR_INVALID_ARG, "dist must be 0 or 1");
+        sfft::check_hip(sfft::fill_synthetic(dst, count, precision == SFFT_F64, dist, seed,
+                                             (hipStream_t)stream),
+                        "fill");
+        return SFFT_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,479 @@
+// engine.cpp -- planner, device tables, plan cache and pass execution.
+#include "engine.h"
+
+#include <sfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+namespace sfft {
+
+void check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw Error(SFFT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const DeviceInfo &device_info() {
+    static DeviceInfo info;
+    static std::once_flag once;
+    std::call_once(once, []() {
+        int dev = 0;
+        check_hip(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t p;
+        check_hip(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+        info.device = dev;
+        info.cu_count = p.multiProcessorCount;
+        info.lds_bytes = p.sharedMemPerBlock;
+        info.arch = p.gcnArchName;
+    });
+    return info;
+}
+
+// ------------------------------------------------------------ size helpers --
+// Same integer steps as the reference (signalsmith-fft.h:317-348): mantissas
+// below 32 in {0,1,2,3,4,6,8,9,12,16,18,24} are accepted.
+
+static bool accepted_mantissa(size_t m) {
+    switch (m) {
+    case 0: case 1: case 2: case 3: case 4: case 6: case 8: case 9: case 12: case 16: case 18:
+    case 24:
+        return true;
+    default:
+        return false;
+    }
+}
+
+size_t size_minimum(size_t n) {
+    size_t scale = 1;
+    for (; n >= 32; scale *= 2) n = (n - 1) / 2 + 1;
+    while (n < 32 && !accepted_mantissa(n)) ++n;
+    return scale * n;
+}
+
+size_t size_maximum(size_t n) {
+    size_t scale = 1;
+    for (; n >= 32; scale *= 2) n /= 2;
+    while (n > 1 && !accepted_mantissa(n)) --n;
+    return scale * n;
+}
+
+// signalsmith-fft.h:403-408 (the maximum is built on sizeMinimum of the half).
+size_t real_size_minimum(size_t n) { return (size_minimum((n - 1) / 2) + 1) * 2; }
+size_t real_size_maximum(size_t n) { return size_minimum(n / 2) * 2; }
+
+// --------------------------------------------------------------- planning --
+
+static const int kRadixSet[] = {16, 15, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+
+static int butterflies_per_thread(int radix, int ept) { return std::max(1, ept / radix); }
+
+// Threads per slot so every stage fits its per-thread butterfly budget.
+static int threads_for(int n, const std::vector<int> &radices, int ept) {
+    int tpt = 1;
+    for (int r : radices) {
+        const int nb = n / r;
+        const int b = butterflies_per_thread(r, ept);
+        tpt = std::max(tpt, (nb + b - 1) / b);
+    }
+    return tpt;
+}
+
+// Cost: sum over stages of thread-slot capacity / work (1.0 = every thread
+// busy), i.e. the number of stages inflated by idle lanes.
+static double radix_cost(int n, const std::vector<int> &radices, int ept, int tpt) {
+    double c = 0;
+    for (int r : radices) c += (double)tpt * butterflies_per_thread(r, ept) * r / n;
+    return c;
+}
+
+static void search_radices(int rem, int max_r, std::vector<int> &cur, int n, int ept,
+                           double &best_cost, std::vector<int> &best) {
+    if (rem == 1) {
+        const int tpt = threads_for(n, cur, ept);
+        if (tpt > 1024) return;
+        const double c = radix_cost(n, cur, ept, tpt);
+        if (c < best_cost - 1e-9) {
+            best_cost = c;
+            best = cur;
+        }
+        return;
+    }
+    if ((int)cur.size() >= kMaxStages) return;
+    for (int r : kRadixSet) {
+        if (r > max_r || rem % r) continue;
+        cur.push_back(r);
+        search_radices(rem / r, r, cur, n, ept, best_cost, best);
+        cur.pop_back();
+    }
+}
+
+bool choose_radices(int n, int ept, std::vector<int> &radices) {
+    radices.clear();
+    if (n < 1) return false;
+    if (n == 1) {
+        radices.push_back(1);
+        return true;
+    }
+    std::vector<int> cur;
+    double best_cost = 1e30;
+    search_radices(n, 1 << 30, cur, n, ept, best_cost, radices);
+    return !radices.empty();
+}
+
+static bool only_small_primes(size_t n) {
+    for (size_t p : {2, 3, 5, 7, 11, 13})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+bool supported_length(size_t n) { return n >= 1 && n <= (1ull << 28) && only_small_primes(n); }
+
+static size_t elem_bytes(int prec) { return prec == kF64 ? 16 : 8; }
+
+static int floor_pow2(int x) {
+    int p = 1;
+    while (p * 2 <= x) p *= 2;
+    return p;
+}
+
+// Fills the stage part of a PassDesc for length n with the given radices.
+static void fill_stages(PassDesc &pd, int n, const std::vector<int> &radices) {
+    pd.n = n;
+    pd.nstages = (int)radices.size();
+    int ns = 1;
+    for (size_t s = 0; s < radices.size(); ++s) {
+        pd.st[s].radix = radices[s];
+        pd.st[s].ns = ns;
+        pd.st[s].tw_off = 0;
+        pd.st[s].div = make_fastdiv((uint32_t)ns);
+        ns *= radices[s];
+    }
+}
+
+// Single-pass layout for sub-transform length n; false if it does not fit.
+static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints *h,
+                          PassPlan &pp) {
+    const DeviceInfo &dev = device_info();
+    const size_t eb = elem_bytes(prec);
+    std::vector<int> radices;
+    int ept = 0;
+    const int epts[2] = {kEptSmall, kEptLarge};
+    for (int e : epts) {
+        if (prec == kF64 && e > kEptSmall) continue;
+        if (h && h->ept && h->ept != e) continue;
+        if (h && !h->radices.empty()) {
+            int prod = 1;
+            for (int r : h->radices) prod *= r;
+            if (prod != n) return false;
+            radices = h->radices;
+            if (threads_for(n, radices, e) > 1024) continue;
+        } else if (!choose_radices(n, e, radices)) {
+            continue;
+        }
+        ept = e;
+        break;
+    }
+    if (!ept) return false;
+    const int tpt = threads_for(n, radices, ept);
+    const int stride = lds_padded(n);
+    const bool uses_lds = uses_lds_always || radices.size() > 1;
+    const size_t slot_lds = uses_lds ? (size_t)stride * eb : 0;
+    if (slot_lds > dev.lds_bytes) return false;
+    int tpw = std::max(1, 256 / tpt);
+    if (h && h->tpw) tpw = h->tpw;
+    if (tpw * tpt > 1024) tpw = std::max(1, 1024 / tpt);
+    if (slot_lds) tpw = std::min<int>(tpw, (int)(dev.lds_bytes / slot_lds));
+    if (tpw < 1) return false;
+    pp = PassPlan{};
+    fill_stages(pp.pd, n, radices);
+    pp.pd.tpt = tpt;
+    pp.pd.tpw = tpw;
+    pp.pd.lds_stride = stride;
+    pp.ept = ept;
+    pp.block = (unsigned)(tpt * tpw);
+    pp.lds_bytes = slot_lds * tpw;
+    return true;
+}
+
+// Column-pass layout: `cols` adjacent slots per block along the contiguous
+// dimension (coalesced column access); cols is capped by LDS and 1024 threads.
+static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, PassPlan &pp) {
+    const DeviceInfo &dev = device_info();
+    const size_t eb = elem_bytes(prec);
+    std::vector<int> radices;
+    int ept = 0;
+    for (int e : {kEptSmall, kEptLarge}) {
+        if (prec == kF64 && e > kEptSmall) continue;
+        if (choose_radices(n, e, radices)) {
+            ept = e;
+            break;
+        }
+    }
+    if (!ept) return false;
+    (void)h;
+    const int tpt = threads_for(n, radices, ept);
+    const int stride = lds_padded(n);
+    const size_t slot_lds = (size_t)stride * eb;
+    int cols = std::min(max_cols, 16);
+    cols = std::min<int>(cols, (int)(dev.lds_bytes / slot_lds));
+    cols = std::min(cols, 1024 / tpt);
+    if (cols < 1) return false;
+    cols = floor_pow2(cols);
+    pp = PassPlan{};
+    fill_stages(pp.pd, n, radices);
+    pp.pd.tpt = tpt;
+    pp.pd.tpw = cols;
+    pp.pd.lds_stride = stride;
+    pp.ept = ept;
+    pp.block = (unsigned)(tpt * cols);
+    pp.lds_bytes = slot_lds * cols;
+    return true;
+}
+
+static int pow2_part(size_t x) {
+    int p = 1;
+    while (x % 2 == 0 && p < (1 << 20)) {
+        x /= 2;
+        p *= 2;
+    }
+    return p;
+}
+
+Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind), prec_(prec) {
+    if (kind != kC2C && kind != kR2C && kind != kR2CHalf)
+        throw Error(SFFT_ERR_INVALID_ARG, "unknown transform kind");
+    if (prec != kF32 && prec != kF64) throw Error(SFFT_ERR_INVALID_ARG, "unknown precision");
+    if (n == 0) throw Error(SFFT_ERR_INVALID_ARG, "size 0");
+    if (kind != kC2C && (n % 2))
+        throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "real transforms need an even size");
+    cn_ = kind == kC2C ? n : n / 2;
+    if (!supported_length(cn_))
+        throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
+                    "size " + std::to_string(n) + ": only factors 2, 3, 5, 7, 11, 13 are planned");
+
+    PassPlan single;
+    const bool force_two = h && h->split > 1;
+    const bool force_one = h && h->split == 1;
+    bool ok = false;
+    if (!force_two && cn_ <= (1u << 20)) ok = layout_single((int)cn_, prec, kind != kC2C, h, single);
+    if (ok) {
+        passes_.push_back(single);
+    } else {
+        if (force_one) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "does not fit one pass");
+        if (kind != kC2C)
+            throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
+                        "real transforms above the single-pass limit are not planned yet");
+        // Two passes: cn = N1 * N2.  Pass 1: N2 column FFTs of length N1
+        // (stride N2) with W_N^{n2 k1}, written transposed as rows; pass 2:
+        // N1 column FFTs of length N2 in place (stride N1).
+        size_t best_n1 = 0;
+        double best_score = 1e30;
+        for (size_t n1 = 32; n1 * 16 <= cn_; ++n1) {
+            if (cn_ % n1) continue;
+            if (h && h->split > 1 && n1 != h->split) continue;
+            const size_t n2 = cn_ / n1;
+            PassPlan p1, p2;
+            if (!layout_columns((int)n1, prec, pow2_part(n2), nullptr, p1)) continue;
+            if (!layout_columns((int)n2, prec, pow2_part(n1), nullptr, p2)) continue;
+            if (p1.pd.nstages < 2) continue;
+            // prefer wide column tiles (coalescing), then balance
+            const double score = -std::log2((double)std::min(p1.pd.tpw, p2.pd.tpw)) * 4.0 +
+                                 std::fabs(std::log2((double)n1) - std::log2((double)n2));
+            if (score < best_score) {
+                best_score = score;
+                best_n1 = n1;
+            }
+        }
+        if (!best_n1) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "no two-pass split");
+        const size_t n1 = best_n1, n2 = cn_ / n1;
+        PassPlan p1, p2;
+        layout_columns((int)n1, prec, pow2_part(n2), nullptr, p1);
+        layout_columns((int)n2, prec, pow2_part(n1), nullptr, p2);
+        p1.slots_per_transform = (int)n2;
+        p1.in_slot_stride = 1;
+        p1.in_elem_stride = (int64_t)n2;
+        p1.out_slot_stride = (int64_t)n1;
+        p1.out_elem_stride = 1;
+        p1.pd.in_col = 1;
+        p1.pd.out_col = 0;
+        p1.post_tw = 1;
+        p2.slots_per_transform = (int)n1;
+        p2.in_from_out = 1;
+        p2.in_slot_stride = 1;
+        p2.in_elem_stride = (int64_t)n1;
+        p2.out_slot_stride = 1;
+        p2.out_elem_stride = (int64_t)n1;
+        p2.pd.in_col = 1;
+        p2.pd.out_col = 1;
+        passes_.push_back(p1);
+        passes_.push_back(p2);
+    }
+    if (prec == kF64)
+        build_tables<double>();
+    else
+        build_tables<float>();
+}
+
+Plan::~Plan() {
+    if (d_tw_) (void)hipFree(d_tw_);
+    if (d_real_tw_) (void)hipFree(d_real_tw_);
+    if (d_rot_) (void)hipFree(d_rot_);
+    if (d_ptw_fine_) (void)hipFree(d_ptw_fine_);
+    if (d_ptw_coarse_) (void)hipFree(d_ptw_coarse_);
+}
+
+template <typename T>
+void Plan::build_tables() {
+    // stage twiddle offsets
+    size_t total = 0;
+    for (auto &p : passes_) {
+        for (int s = 0; s < p.pd.nstages; ++s) {
+            StageDesc &st = p.pd.st[s];
+            if (st.ns > 1) {
+                st.tw_off = (int32_t)total;
+                total += (size_t)st.ns * (st.radix - 1);
+            }
+        }
+    }
+    tw_count_ = total;
+    hipStream_t s;
+    check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{s};
+    auto alloc = [&](void **p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess)
+            throw Error(SFFT_ERR_OUT_OF_MEMORY, "device allocation of plan tables failed");
+        table_bytes_ += bytes;
+    };
+    if (total) {
+        alloc(&d_tw_, total * sizeof(C2<T>));
+        for (auto &p : passes_)
+            for (int st = 0; st < p.pd.nstages; ++st)
+                if (p.pd.st[st].ns > 1)
+                    check_hip(gen_stage_twiddles<T>((C2<T> *)d_tw_, p.pd.st[st].tw_off,
+                                                    p.pd.st[st].ns, p.pd.st[st].radix, s),
+                              "twiddle generation");
+    }
+    if (kind_ != kC2C) {
+        const size_t nreal = n_;
+        alloc(&d_real_tw_, (nreal / 4 + 1) * sizeof(C2<T>));
+        if (kind_ == kR2CHalf) alloc(&d_rot_, (nreal / 2) * sizeof(C2<T>));
+        check_hip(gen_real_twiddles<T>((C2<T> *)d_real_tw_, (C2<T> *)d_rot_, nreal,
+                                       kind_ == kR2CHalf, s),
+                  "real twiddle generation");
+    }
+    if (passes_.size() > 1) {
+        int shift = 0;
+        while ((1ull << (2 * shift)) < cn_) ++shift;
+        ptw_shift_ = shift;
+        const size_t nf = 1ull << shift, nc = (cn_ + nf - 1) >> shift;
+        alloc((void **)&d_ptw_fine_, nf * sizeof(C2<double>));
+        alloc((void **)&d_ptw_coarse_, nc * sizeof(C2<double>));
+        check_hip(gen_post_twiddles(d_ptw_fine_, d_ptw_coarse_, shift, cn_, s),
+                  "post twiddle generation");
+    }
+    check_hip(hipStreamSynchronize(s), "plan table sync");
+}
+
+size_t Plan::default_in_dist(int inverse) const {
+    if (kind_ == kC2C) return n_;
+    return inverse ? cn_ : n_;
+}
+size_t Plan::default_out_dist(int inverse) const {
+    if (kind_ == kC2C) return n_;
+    return inverse ? n_ : cn_;
+}
+
+size_t Plan::in_bytes(size_t batch, int inverse) const {
+    const size_t rb = prec_ == kF64 ? 8 : 4;
+    if (kind_ == kC2C || inverse) return batch * cn_ * 2 * rb;
+    return batch * n_ * rb;
+}
+size_t Plan::out_bytes(size_t batch, int inverse) const {
+    const size_t rb = prec_ == kF64 ? 8 : 4;
+    if (kind_ == kC2C || !inverse) return batch * cn_ * 2 * rb;
+    return batch * n_ * rb;
+}
+
+template <typename T>
+void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
+               int inverse, hipStream_t stream) const {
+    // distances in complex elements
+    int64_t din = (int64_t)in_dist, dout = (int64_t)out_dist;
+    if (kind_ != kC2C) {
+        if (inverse) {
+            if (out_dist % 2) throw Error(SFFT_ERR_INVALID_ARG, "real output distance must be even");
+            dout = (int64_t)(out_dist / 2);
+        } else {
+            if (in_dist % 2) throw Error(SFFT_ERR_INVALID_ARG, "real input distance must be even");
+            din = (int64_t)(in_dist / 2);
+        }
+    }
+    const size_t np = passes_.size();
+    for (size_t p = 0; p < np; ++p) {
+        const PassPlan &pp = passes_[p];
+        const size_t slots = batch * (size_t)pp.slots_per_transform;
+        if (slots > 0x7fffffffull) throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
+        PassArgs<T> a{};
+        a.pd = pp.pd;
+        const FastDiv pb = make_fastdiv((uint32_t)pp.slots_per_transform);
+        a.iin = IoDesc{pp.in_from_out ? dout : din, pp.in_slot_stride, pp.in_elem_stride, pb};
+        a.iout = IoDesc{dout, pp.out_slot_stride, pp.out_elem_stride, pb};
+        a.in = pp.in_from_out ? (const C2<T> *)out : (const C2<T> *)in;
+        a.out = (C2<T> *)out;
+        a.tw = (const C2<T> *)d_tw_;
+        a.total_slots = (int32_t)slots;
+        a.conj_in = inverse && p == 0;
+        a.conj_out = inverse && p + 1 == np;
+        a.mode = kind_ == kC2C ? kModeC2C : (inverse ? kModeC2R : kModeR2C);
+        a.half_shift = kind_ == kR2CHalf;
+        a.post_tw = pp.post_tw;
+        a.post_tw_shift = ptw_shift_;
+        a.real_tw = (const C2<T> *)d_real_tw_;
+        a.rot = (const C2<T> *)d_rot_;
+        a.ptw_fine = d_ptw_fine_;
+        a.ptw_coarse = d_ptw_coarse_;
+        const unsigned grid = (unsigned)((slots + pp.pd.tpw - 1) / pp.pd.tpw);
+        check_hip(launch_pass<T>(a, pp.ept, grid, pp.block, pp.lds_bytes, stream), "pass launch");
+    }
+}
+
+void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
+                   int inverse, hipStream_t stream) const {
+    if (batch == 0) return;
+    if (!in || !out) throw Error(SFFT_ERR_INVALID_ARG, "null buffer");
+    // overlap check on the byte ranges actually touched
+    const size_t rb = prec_ == kF64 ? 8 : 4;
+    const size_t in_unit = (kind_ == kC2C || inverse) ? 2 * rb : rb;
+    const size_t out_unit = (kind_ == kC2C || !inverse) ? 2 * rb : rb;
+    const size_t in_len = ((batch - 1) * in_dist + default_in_dist(inverse)) * in_unit;
+    const size_t out_len = ((batch - 1) * out_dist + default_out_dist(inverse)) * out_unit;
+    const uintptr_t i0 = (uintptr_t)in, o0 = (uintptr_t)out;
+    if (i0 < o0 + out_len && o0 < i0 + in_len)
+        throw Error(SFFT_ERR_ALIASING, "input and output overlap (transforms are out of place)");
+    if (in_dist < default_in_dist(inverse) || out_dist < default_out_dist(inverse))
+        throw Error(SFFT_ERR_INVALID_ARG, "batch distance shorter than one transform");
+    if (prec_ == kF64)
+        run<double>(in, in_dist, out, out_dist, batch, inverse, stream);
+    else
+        run<float>(in, in_dist, out, out_dist, batch, inverse, stream);
+}
+
+std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec) {
+    static std::mutex mu;
+    static std::map<std::tuple<size_t, int, int>, std::shared_ptr<Plan>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple(n, kind, prec);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    auto p = std::make_shared<Plan>(n, kind, prec);
+    cache[key] = p;
+    return p;
+}
+
+}  // namespace sfft

@@ -1,0 +1,120 @@
+// engine.h -- host side of the transform engine: planner, device tables,
+// plan cache and pass execution.  The C ABI (capi.cpp) is a thin layer on top.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "plan_desc.h"
+
+namespace sfft {
+
+enum Kind : int { kC2C = 0, kR2C = 1, kR2CHalf = 2 };
+enum Prec : int { kF32 = 0, kF64 = 1 };
+
+// Error with an ABI status code.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+struct DeviceInfo {
+    int device = 0;
+    int cu_count = 256;
+    size_t lds_bytes = 160 * 1024;
+    std::string arch;
+};
+const DeviceInfo &device_info();
+
+// Integer-exact size helpers (reference signalsmith-fft.h:317-348, :403-408).
+size_t size_minimum(size_t n);
+size_t size_maximum(size_t n);
+size_t real_size_minimum(size_t n);
+size_t real_size_maximum(size_t n);
+
+// One launch over `slots` sub-transforms.
+struct PassPlan {
+    PassDesc pd{};
+    int ept = kEptSmall;
+    int slots_per_transform = 1;  // per_batch of the slot -> address map
+    int in_from_out = 0;          // reads the output buffer (second pass)
+    int post_tw = 0;
+    // address map; batch strides are filled at execute time
+    int64_t in_slot_stride = 0, in_elem_stride = 1;
+    int64_t out_slot_stride = 0, out_elem_stride = 1;
+    unsigned block = 0;
+    size_t lds_bytes = 0;
+};
+
+// Overrides the planner's choice (autotuner candidates).
+struct PlanHints {
+    std::vector<int> radices;  // first-pass radix order (empty: planner's)
+    int tpw = 0;               // slots per block (0: planner's)
+    int ept = 0;               // elements-per-thread instance (0: planner's)
+    size_t split = 0;          // N1 of a two-pass plan (0: planner's; 1: force single pass)
+};
+
+class Plan {
+  public:
+    Plan(size_t n, int kind, int prec, const PlanHints *hints = nullptr);
+    ~Plan();
+    Plan(const Plan &) = delete;
+    Plan &operator=(const Plan &) = delete;
+
+    size_t n() const { return n_; }
+    int kind() const { return kind_; }
+    int prec() const { return prec_; }
+    size_t complex_len() const { return cn_; }
+    const std::vector<PassPlan> &passes() const { return passes_; }
+    size_t table_bytes() const { return table_bytes_; }
+
+    // in/out: device pointers; dists in elements of the in/out type
+    // (complex for C2C, real/complex for the real kinds).
+    void execute(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
+                 int inverse, hipStream_t stream) const;
+
+    // Default distances for a contiguous batch.
+    size_t default_in_dist(int inverse) const;
+    size_t default_out_dist(int inverse) const;
+    size_t in_bytes(size_t batch, int inverse) const;
+    size_t out_bytes(size_t batch, int inverse) const;
+
+  private:
+    template <typename T>
+    void build_tables();
+    template <typename T>
+    void run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch, int inverse,
+             hipStream_t stream) const;
+
+    size_t n_;
+    int kind_, prec_;
+    size_t cn_;  // complex FFT length (n, or n/2 for real kinds)
+    std::vector<PassPlan> passes_;
+    std::vector<std::vector<int>> stage_ns_;  // per pass, per stage span (for tables)
+    void *d_tw_ = nullptr;
+    size_t tw_count_ = 0;
+    void *d_real_tw_ = nullptr;
+    void *d_rot_ = nullptr;
+    C2<double> *d_ptw_fine_ = nullptr;
+    C2<double> *d_ptw_coarse_ = nullptr;
+    int ptw_shift_ = 0;
+    size_t table_bytes_ = 0;
+};
+
+// Cached plans keyed by (n, kind, precision).
+std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec);
+
+// Radix choice for a single pass of length n (exposed for tests/tuning).
+bool choose_radices(int n, int ept, std::vector<int> &radices);
+bool supported_length(size_t n);
+
+void check_hip(hipError_t e, const char *what);
+
+}  // namespace sfft

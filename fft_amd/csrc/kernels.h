@@ -1,0 +1,39 @@
+// kernels.h -- host-side launch helpers for the device code in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "stockham.h"
+
+namespace sfft {
+
+// Elements-per-thread instances compiled for the pass kernel.
+constexpr int kEptSmall = 16;
+constexpr int kEptLarge = 32;
+
+template <typename T>
+hipError_t launch_pass(const PassArgs<T> &a, int ept, unsigned grid, unsigned block,
+                       size_t lds_bytes, hipStream_t stream);
+
+// Stage twiddles W_{ns R}^{r k} for k < ns, r = 1..R-1 at dst[off + k (R-1) + r - 1],
+// computed in FP64 (sincospi) and rounded once to T.
+template <typename T>
+hipError_t gen_stage_twiddles(C2<T> *dst, int off, int ns, int radix, hipStream_t stream);
+
+// twiddlesMinusI[i] = -i e^{-2 pi i (i + s) / n}, i <= n/4, s = 1/2 for the
+// half-shift kind (signalsmith-fft.h:420-425), and the half-shift
+// rotations rot[i] = e^{-2 pi i * i / n}, i < n/2 (:426-432).
+template <typename T>
+hipError_t gen_real_twiddles(C2<T> *tmi, C2<T> *rot, size_t n, int half_shift,
+                             hipStream_t stream);
+
+// Two-level FP64 table for W_N^m: fine[m] for m < 2^shift,
+// coarse[q] = W_N^{q 2^shift} for q < ceil(N / 2^shift).
+hipError_t gen_post_twiddles(C2<double> *fine, C2<double> *coarse, int shift, size_t N,
+                             hipStream_t stream);
+
+// Counter-based splitmix64 inputs (same stream as fft_amd/synth.py).
+hipError_t fill_synthetic(void *dst, size_t count, int f64, int dist, uint64_t seed,
+                          hipStream_t stream);
+
+}  // namespace sfft

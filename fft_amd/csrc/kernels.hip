@@ -1,0 +1,174 @@
+// kernels.hip -- pass kernel, FP64 table generators and the synthetic input
+// generator for gfx950.
+#include "kernels.h"
+
+namespace sfft {
+
+// ---------------------------------------------------------------- passes --
+
+template <typename T, int EPT>
+__global__ void __launch_bounds__(1024) k_pass(PassArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
+    Block<T, EPT> blk(a, lds, (int)threadIdx.x, (int)blockIdx.x * a.pd.tpw);
+    blk.run();
+}
+
+template <typename T, int EPT>
+static hipError_t launch_one(const PassArgs<T> &a, unsigned grid, unsigned block, size_t lds_bytes,
+                             hipStream_t stream) {
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute((const void *)k_pass<T, EPT>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_pass<T, EPT>), dim3(grid), dim3(block), lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+template <>
+hipError_t launch_pass<float>(const PassArgs<float> &a, int ept, unsigned grid, unsigned block,
+                              size_t lds_bytes, hipStream_t stream) {
+    if (grid == 0) return hipSuccess;
+    if (ept <= kEptSmall) return launch_one<float, kEptSmall>(a, grid, block, lds_bytes, stream);
+    return launch_one<float, kEptLarge>(a, grid, block, lds_bytes, stream);
+}
+
+// FP64 has only the 16-element instance (32 elements of double2 spill).
+template <>
+hipError_t launch_pass<double>(const PassArgs<double> &a, int ept, unsigned grid, unsigned block,
+                               size_t lds_bytes, hipStream_t stream) {
+    if (grid == 0) return hipSuccess;
+    if (ept > kEptSmall) return hipErrorInvalidValue;
+    return launch_one<double, kEptSmall>(a, grid, block, lds_bytes, stream);
+}
+
+
+// --------------------------------------------------------------- twiddles --
+
+// e^{-2 pi i m / L} for an exact integer ratio, in FP64.
+__device__ __forceinline__ C2<double> unit_root(uint64_t m, uint64_t L) {
+    m %= L;
+    double s, c;
+    sincospi(-2.0 * (double)m / (double)L, &s, &c);
+    return {c, s};
+}
+
+template <typename T>
+__global__ void k_stage_tw(C2<T> *dst, int off, int ns, int radix) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cnt = ns * (radix - 1);
+    if (idx >= cnt) return;
+    const int k = idx / (radix - 1);
+    const int r = idx % (radix - 1) + 1;
+    const C2<double> w = unit_root((uint64_t)r * (uint64_t)k, (uint64_t)ns * (uint64_t)radix);
+    dst[off + idx] = {(T)w.x, (T)w.y};
+}
+
+template <typename T>
+hipError_t gen_stage_twiddles(C2<T> *dst, int off, int ns, int radix, hipStream_t stream) {
+    const int cnt = ns * (radix - 1);
+    if (cnt <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_stage_tw<T>), dim3((cnt + 255) / 256), dim3(256), 0, stream, dst, off,
+                       ns, radix);
+    return hipGetLastError();
+}
+template hipError_t gen_stage_twiddles<float>(C2<float> *, int, int, int, hipStream_t);
+template hipError_t gen_stage_twiddles<double>(C2<double> *, int, int, int, hipStream_t);
+
+template <typename T>
+__global__ void k_real_tw(C2<T> *tmi, C2<T> *rot, uint64_t n, int half_shift) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i <= n / 4) {
+        // -i e^{-2 pi i (i + s)/n} with s = 1/2: exponent over 2n is (2i + 1).
+        const C2<double> w = half_shift ? unit_root(2 * i + 1, 2 * n) : unit_root(i, n);
+        tmi[i] = {(T)w.y, (T)(-w.x)};
+    }
+    if (half_shift && i < n / 2) {
+        const C2<double> w = unit_root(i, n);
+        rot[i] = {(T)w.x, (T)w.y};
+    }
+}
+
+template <typename T>
+hipError_t gen_real_twiddles(C2<T> *tmi, C2<T> *rot, size_t n, int half_shift,
+                             hipStream_t stream) {
+    const size_t cnt = n / 2 + 1;
+    hipLaunchKernelGGL((k_real_tw<T>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream,
+                       tmi, rot, (uint64_t)n, half_shift);
+    return hipGetLastError();
+}
+template hipError_t gen_real_twiddles<float>(C2<float> *, C2<float> *, size_t, int, hipStream_t);
+template hipError_t gen_real_twiddles<double>(C2<double> *, C2<double> *, size_t, int,
+                                              hipStream_t);
+
+__global__ void k_post_tw(C2<double> *fine, C2<double> *coarse, int shift, uint64_t N) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t nf = 1ull << shift;
+    const uint64_t nc = (N + nf - 1) >> shift;
+    if (i < nf) fine[i] = unit_root(i, N);
+    if (i < nc) coarse[i] = unit_root(i << shift, N);
+}
+
+hipError_t gen_post_twiddles(C2<double> *fine, C2<double> *coarse, int shift, size_t N,
+                             hipStream_t stream) {
+    const size_t nf = 1ull << shift;
+    const size_t nc = (N + nf - 1) >> shift;
+    const size_t cnt = nf > nc ? nf : nc;
+    hipLaunchKernelGGL(k_post_tw, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, fine,
+                       coarse, shift, (uint64_t)N);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------- synthetic inputs --
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void k_fill(T *dst, uint64_t count, int dist, uint64_t key) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += stride) {
+        if (dist == 0) {
+            const uint64_t w = mix64(key + (i + 1) * 0x9E3779B97F4A7C15ull);
+            const double u = (double)(w >> 40) * (1.0 / 16777216.0);
+            dst[i] = (T)(2.0 * u - 1.0);
+        } else {
+            // Box-Muller on words (2p, 2p+1) of pair p = i/2; even i takes cos, odd i sin.
+            const uint64_t p = i >> 1;
+            const uint64_t w1 = mix64(key + (2 * p + 1) * 0x9E3779B97F4A7C15ull);
+            const uint64_t w2 = mix64(key + (2 * p + 2) * 0x9E3779B97F4A7C15ull);
+            const double u1 = ((double)(w1 >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+            const double u2 = (double)(w2 >> 11) * (1.0 / 9007199254740992.0);
+            const double r = sqrt(-2.0 * log(u1));
+            double s, c;
+            sincospi(2.0 * u2, &s, &c);
+            dst[i] = (T)((i & 1) ? r * s : r * c);
+        }
+    }
+}
+
+hipError_t fill_synthetic(void *dst, size_t count, int f64, int dist, uint64_t seed,
+                          hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    const uint64_t key = [](uint64_t z) {
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }(seed);
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (f64)
+        hipLaunchKernelGGL((k_fill<double>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                           (double *)dst, (uint64_t)count, dist, key);
+    else
+        hipLaunchKernelGGL((k_fill<float>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                           (float *)dst, (uint64_t)count, dist, key);
+    return hipGetLastError();
+}
+
+}  // namespace sfft

@@ -1,0 +1,156 @@
+/*
+ * sfft.h -- C ABI of the MI355X transform engine (libsfft.so).
+ *
+ * The reference library is the header-only C++ template signalsmith-fft.h;
+ * it has no C ABI.  These entry points are what a binding for its transform
+ * path needs; each one names the reference interface it stands in for.
+ * The C++ drop-in facade include/signalsmith-fft.h is built on top of them.
+ *
+ * Conventions
+ *  - Complex data is interleaved (re, im): 8 bytes per value in F32, 16 in F64.
+ *  - A batch is `batch` transforms stored back to back (stride n complex
+ *    values for C2C, n reals / n/2 complex values for the real kinds).
+ *  - All data pointers passed to sfft_execute* are DEVICE pointers (HBM).
+ *  - Transforms are out of place; `in` is never written.  in == out is
+ *    rejected with SFFT_ERR_ALIASING (the reference silently corrupts).
+ *  - The inverse is unnormalised: ifft(fft(x)) = n * x (reference
+ *    signalsmith-fft.h:295-315, no 1/N scaling).
+ *  - Streams are hipStream_t passed as void*; NULL is the default stream.
+ *  - Every function returns SFFT_OK (0) or a negative error code;
+ *    sfft_last_error() gives a message for the calling thread.
+ *  - A plan is immutable after creation and may be shared, but executions
+ *    of one plan on several streams at once are allowed only for plans whose
+ *    sfft_plan_info.workspace_bytes is 0 (every plan this library builds).
+ */
+#ifndef SFFT_H
+#define SFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFFT_VERSION_MAJOR 0
+#define SFFT_VERSION_MINOR 1
+
+/* status codes */
+#define SFFT_OK 0
+#define SFFT_ERR_INVALID_ARG (-1)
+#define SFFT_ERR_UNSUPPORTED_SIZE (-2)
+#define SFFT_ERR_OUT_OF_MEMORY (-3)
+#define SFFT_ERR_DEVICE (-4)
+#define SFFT_ERR_ALIASING (-5)
+#define SFFT_ERR_NOT_READY (-6)
+#define SFFT_ERR_BUDGET (-7)
+
+/* transform kinds */
+#define SFFT_C2C 0           /* FFT<V>                       (signalsmith-fft.h:69-387)  */
+#define SFFT_R2C 1           /* RealFFT<V>                   (signalsmith-fft.h:393-503) */
+#define SFFT_R2C_HALFSHIFT 2 /* RealFFT<V, halfFreqShift> = ModifiedRealFFT<V> (:389-391, :505-508) */
+
+/* precisions (the arithmetic type of the transform) */
+#define SFFT_F32 0 /* FFT<float>  */
+#define SFFT_F64 1 /* FFT<double> */
+
+/* directions */
+#define SFFT_FORWARD 0 /* fft()  */
+#define SFFT_INVERSE 1 /* ifft() */
+
+typedef struct sfft_plan_s *sfft_plan;
+
+typedef struct {
+    size_t n;               /* transform length (reals for the real kinds) */
+    int kind, precision;
+    int passes;             /* HBM passes per transform (1 = LDS resident) */
+    int nradices;           /* radices of the first pass, in stage order */
+    int radices[16];
+    int threads_per_transform, transforms_per_block;
+    size_t workspace_bytes; /* device scratch held by the plan (0: none) */
+    size_t table_bytes;     /* device twiddle tables held by the plan */
+} sfft_plan_info;
+
+/* ---- library ---- */
+
+/* Initialise the HIP device (optional; the first plan does it). */
+int sfft_init(void);
+/* Message for the last error on this thread ("" if none). */
+const char *sfft_last_error(void);
+/* Device properties the planner uses: CU count, LDS bytes per block. */
+int sfft_device_info(int *cu_count, size_t *lds_bytes, char *arch, size_t arch_len);
+
+/* ---- size helpers: integer-exact restatements of the reference ---- */
+
+/* FFT<V>::sizeMinimum / sizeMaximum (signalsmith-fft.h:327-348): fast size
+ * 2^a * {1,3,9} >= n / <= n. */
+size_t sfft_size_minimum(size_t n);
+size_t sfft_size_maximum(size_t n);
+/* RealFFT<V>::sizeMinimum / sizeMaximum (signalsmith-fft.h:403-408),
+ * including the reference's quirks (sizeMaximum may exceed n). */
+size_t sfft_real_size_minimum(size_t n);
+size_t sfft_real_size_maximum(size_t n);
+/* 1 if the engine plans this length for the kind, else 0. */
+int sfft_size_supported(size_t n, int kind);
+
+/* ---- plans ---- */
+
+/* FFT<V>::FFT(size) / setSize(size) and RealFFT<V,flags>::setSize
+ * (signalsmith-fft.h:350-363, :410-435): factorise, choose radices, build the
+ * twiddle tables on the device in FP64 and round them once.  Plans are
+ * cached by (n, kind, precision); creating the same plan again is cheap. */
+int sfft_plan_create(sfft_plan *plan, size_t n, int kind, int precision);
+int sfft_plan_destroy(sfft_plan plan);
+int sfft_plan_get_info(sfft_plan plan, sfft_plan_info *info);
+
+/* FFT<V>::fft / ifft and RealFFT<V,flags>::fft / ifft
+ * (signalsmith-fft.h:374-386, :446-502) over a batch of device buffers.
+ *   C2C:       in, out: batch x n complex
+ *   R2C fwd:   in: batch x n reals;  out: batch x n/2 complex,
+ *              out[0] = (DC, Nyquist) packed (not for the half-shift kind)
+ *   R2C inv:   in: batch x n/2 complex; out: batch x n reals
+ * batch == 0 is a no-op.  The call is asynchronous on `stream`. */
+int sfft_execute(sfft_plan plan, const void *in, void *out, size_t batch, int direction,
+                 void *stream);
+
+/* Same as sfft_execute with an explicit distance (in elements of the
+ * input / output type) between consecutive transforms of the batch. */
+int sfft_execute_strided(sfft_plan plan, const void *in, size_t in_dist, void *out,
+                         size_t out_dist, size_t batch, int direction, void *stream);
+
+/* ---- uneven batches: many sizes, one persistent launch ---- */
+
+typedef struct sfft_batch_s *sfft_batch;
+
+typedef struct {
+    size_t n;          /* transform length */
+    int kind;          /* SFFT_C2C, SFFT_R2C or SFFT_R2C_HALFSHIFT */
+    int direction;     /* SFFT_FORWARD / SFFT_INVERSE */
+    size_t count;      /* transforms of this shape, back to back */
+    size_t in_offset;  /* byte offset of the first input in the input arena */
+    size_t out_offset; /* byte offset of the first output in the output arena */
+} sfft_batch_item;
+
+/* Builds the work list and the per-size plans (no device work per item at
+ * execute time beyond the one launch). */
+int sfft_batch_create(sfft_batch *batch, const sfft_batch_item *items, size_t count,
+                      int precision);
+int sfft_batch_destroy(sfft_batch batch);
+/* One persistent launch over every item: in_arena/out_arena are device
+ * base pointers for the item offsets. */
+int sfft_batch_execute(sfft_batch batch, const void *in_arena, void *out_arena, void *stream);
+/* Same work as one sfft_execute per item group (the plain per-size loop). */
+int sfft_batch_execute_per_size(sfft_batch batch, const void *in_arena, void *out_arena,
+                                void *stream);
+
+/* ---- synthetic inputs (seeded, counter-based splitmix64) ---- */
+
+/* dist: 0 = U[-1, 1), 1 = N(0, 1) (Box-Muller); count values of the
+ * precision's real type (complex data: 2 per element). */
+int sfft_fill(void *dst, size_t count, int precision, int dist, uint64_t seed, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFFT_H */

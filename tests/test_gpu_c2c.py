@@ -1,0 +1,217 @@
+"""GPU parity of the complex transform (FFT<V>) through the C ABI.
+
+Bar: FP64 matches the reference's FFT<double> golden vectors to rel L2 1e-13;
+FP32 matches the FP64 oracle to rel L2 1e-6 (the tolerance of the transform
+path; the reference's own FFT<float> sits at 1.6e-7..5.3e-7); integer size
+behaviour is bit-exact.  Reference tests re-expressed: tests/00-fft.cpp.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import fft_amd
+from fft_amd import _lib as L
+from fft_amd import synth
+from oracle import pyoracle as po
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-6   # rel L2 vs the FP64 oracle, FP32 path
+TOL64 = 1e-13  # rel L2 vs reference FFT<double>, FP64 path
+
+TEST_SIZES = [1, 2, 4, 8, 16, 32, 64, 128, 256, 3, 6, 9, 12, 18, 24, 5, 10, 15, 20, 25,
+              7, 14, 21, 28, 49, 11, 13, 22]  # tests/00-fft.cpp:8-16 (17, 19, 23 below)
+
+
+def _dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _run(n, x, inverse=False, dtype=torch.complex64):
+    f = fft_amd.FFT(n, dtype=dtype)
+    xd = _dev(x)
+    yd = torch.empty_like(xd)
+    (f.ifft if inverse else f.fft)(xd, yd)
+    torch.cuda.synchronize()
+    return yd.cpu().numpy()
+
+
+def fast_sizes(lo, hi):
+    out = []
+    p2 = 1
+    while p2 <= hi:
+        p3 = p2
+        while p3 <= hi:
+            p5 = p3
+            while p5 <= hi:
+                if p5 >= lo:
+                    out.append(p5)
+                p5 *= 5
+            p3 *= 3
+        p2 *= 2
+    return sorted(out)
+
+
+def test_golden_fft_double():
+    g = np.load(os.path.join(GOLDEN, "c2c_f64.npz"))
+    sizes = sorted(int(k[3:]) for k in g.files if k.startswith("in_"))
+    checked = 0
+    for n in sizes:
+        if not L.lib().sfft_size_supported(n, L.C2C):
+            continue
+        x = g[f"in_{n}"]
+        y = _run(n, x, dtype=torch.complex128)
+        z = _run(n, x, inverse=True, dtype=torch.complex128)
+        assert po.rel_l2(y, g[f"fwd_{n}"]) <= TOL64, n
+        assert po.rel_l2(z, g[f"inv_{n}"]) <= TOL64, n
+        checked += 1
+    assert checked >= 38
+
+
+def test_float_error_level_vs_reference_float():
+    g = np.load(os.path.join(GOLDEN, "c2c_f32.npz"))
+    for n in (1920, 3000, 4096):
+        x = g[f"in_{n}"]
+        exact = po.fft(x.astype(np.complex128))
+        err_gpu = po.rel_l2(_run(n, x), exact)
+        err_ref = po.rel_l2(g[f"fwd_{n}"], exact)
+        print(f"N={n}: GPU FFT<float> rel L2 {err_gpu:.2e}, reference FFT<float> {err_ref:.2e}")
+        assert err_gpu <= TOL32
+        assert err_gpu <= 2 * err_ref
+
+
+@pytest.mark.parametrize("n", TEST_SIZES)
+def test_individual_bins(n):
+    # tests/00-fft.cpp:19-60, FFT<double>, all bins in one batch
+    t = np.arange(n)
+    x = np.exp(2j * np.pi * np.outer(np.arange(n), t) / n)  # row b = bin b
+    keep = x.copy()
+    f = fft_amd.FFT(n, dtype=torch.complex128)
+    xd = _dev(x)
+    yd = torch.empty_like(xd)
+    f.fft(xd, yd)
+    torch.cuda.synchronize()
+    assert np.array_equal(xd.cpu().numpy(), keep), "input was changed"
+    y = yd.cpu().numpy()
+    expect = n * np.eye(n)
+    for b in range(n):
+        assert po.close_enough_metric(y[b], expect[b]) < 1e-6, (n, b)
+        assert np.max(np.abs(y[b] - expect[b])) <= 1e-9 * n
+
+
+@pytest.mark.parametrize("n", TEST_SIZES)
+def test_linearity_and_inverse(n):
+    # tests/00-fft.cpp:62-145
+    a = synth.complex_uniform(1000 + n, (n,), np.complex128) * 0.5
+    b = synth.complex_uniform(2000 + n, (n,), np.complex128) * 0.5
+    fa = _run(n, a, dtype=torch.complex128)
+    fb = _run(n, b, dtype=torch.complex128)
+    fab = _run(n, a + b, dtype=torch.complex128)
+    assert po.close_enough_metric(fab, fa + fb) < 1e-6
+    assert po.rel_l2(fab, fa + fb) < 1e-13
+    back = _run(n, fa, inverse=True, dtype=torch.complex128)
+    assert po.rel_l2(back, n * a) < 1e-13
+    h1 = np.exp(2j * np.pi * np.arange(n) / n)
+    back1 = _run(n, _run(n, h1, dtype=torch.complex128), inverse=True, dtype=torch.complex128)
+    assert po.close_enough_metric(back1, n * h1) < 1e-6
+
+
+@pytest.mark.parametrize("n", [17, 19, 23])
+def test_large_prime_sizes_rejected_with_status(n):
+    assert L.lib().sfft_size_supported(n, L.C2C) == 0
+    with pytest.raises(L.SfftError) as e:
+        fft_amd.FFT(n)
+    assert e.value.code == L.SFFT_ERR_UNSUPPORTED_SIZE
+
+
+def test_all_fast_sizes_float():
+    # every 2^a 3^b 5^c in [2, 65536] (the uneven-batch size set), batch 2
+    sizes = fast_sizes(2, 65536)
+    worst = 0.0
+    for n in sizes:
+        x = synth.complex_uniform(n, (2, n), np.complex64)
+        y = _run(n, x)
+        err = po.rel_l2(y, po.fft(x.astype(np.complex128)))
+        worst = max(worst, err)
+        assert err <= TOL32, (n, err)
+    print(f"{len(sizes)} sizes, worst rel L2 {worst:.2e}")
+
+
+def test_all_fast_sizes_double_inverse():
+    sizes = fast_sizes(2, 8192)
+    for n in sizes:
+        x = synth.complex_uniform(n, (1, n), np.complex128)
+        y = _run(n, x, inverse=True, dtype=torch.complex128)
+        assert po.rel_l2(y, po.ifft(x)) <= TOL64, n
+
+
+@pytest.mark.parametrize("n", [1920, 3000, 16384])
+def test_batched_config_sizes(n):
+    # the batched config: 8192 transforms of N(0,1) complex64 per size
+    batch = 8192
+    x = synth.complex_normal(n, (batch, n), np.complex64)
+    f = fft_amd.FFT(n)
+    xd = _dev(x)
+    yd = torch.empty_like(xd)
+    zd = torch.empty_like(xd)
+    f.fft(xd, yd)
+    f.ifft(yd, zd)
+    torch.cuda.synchronize()
+    rows = np.linspace(0, batch - 1, 12).astype(int)
+    y = yd[rows].cpu().numpy()
+    err = po.rel_l2(y, po.fft(x[rows].astype(np.complex128)))
+    assert err <= TOL32, err
+    # inverse against the oracle on its own input
+    zi = _run(n, y, inverse=True)
+    assert po.rel_l2(zi, po.ifft(y.astype(np.complex128))) <= TOL32
+    # whole batch: round trip = N x
+    rt = (zd.double() / n - xd.to(torch.complex128)).abs().pow(2).sum().sqrt() / \
+        xd.to(torch.complex128).abs().pow(2).sum().sqrt()
+    assert rt.item() <= 2 * TOL32, rt.item()
+    # a row alone equals the row in the batch, bitwise
+    one = _run(n, x[rows[5]][None, :])
+    assert np.array_equal(one[0], y[5])
+
+
+@pytest.mark.parametrize("n,batch", [(32768, 4), (65536, 3), (3 * 2 ** 15, 2), (2 ** 18, 2),
+                                     (2 ** 20, 2), (2 ** 24, 1)])
+def test_two_pass_sizes(n, batch):
+    x = synth.complex_normal(n + 7, (batch, n), np.complex64)
+    f = fft_amd.FFT(n)
+    assert f.plan.info()["passes"] == 2
+    y = _run(n, x)
+    err = po.rel_l2(y, po.fft(x.astype(np.complex128)))
+    assert err <= TOL32, err
+    z = _run(n, y, inverse=True)
+    assert po.rel_l2(z, po.ifft(y.astype(np.complex128))) <= TOL32
+
+
+def test_two_pass_double():
+    n = 2 ** 16
+    x = synth.complex_uniform(5, (2, n), np.complex128)
+    assert po.rel_l2(_run(n, x, dtype=torch.complex128), po.fft(x)) <= 1e-12
+
+
+def test_aliasing_rejected():
+    f = fft_amd.FFT(64)
+    x = torch.zeros(4, 64, dtype=torch.complex64, device="cuda")
+    with pytest.raises(L.SfftError) as e:
+        f.fft(x, x)
+    assert e.value.code == L.SFFT_ERR_ALIASING
+
+
+def test_setters_mirror_reference():
+    t = np.load(os.path.join(GOLDEN, "sizes.npz"))
+    f = fft_amd.FFT(1)
+    for n in (5, 100, 1000, 1025, 4000):
+        assert f.setSizeMinimum(n) == t["fft_setmin"][n - 1]
+        assert f.setSizeMaximum(n) == t["fft_setmax"][n - 1]
+        assert f.setSize(n if L.lib().sfft_size_supported(n, L.C2C) else 64) in (n, 64)
+    for d in (-1, 0, 1):
+        for n in (7, 64, 1000, 3001):
+            if d == 0 and not L.lib().sfft_size_supported(n, L.C2C):
+                continue
+            assert fft_amd.FFT(n, d).size() == t[f"fft_ctor_{d + 1}"][n - 1]
