@@ -1,0 +1,283 @@
+"""Benchmark of the batched FP32 transform path on one MI355X.
+
+Workload (one step): complex FFT<float> N in {1920, 3000, 16384} x 8192
+transforms (N(0,1) inputs), forward and inverse, plus RealFFT<float> N=2048 over
+16384 real frames (8 sinusoids + 0.01 N(0,1)), forward and inverse.  Inputs
+are generated in HBM from fixed seeds before timing; every launch reads and
+writes HBM.
+
+value  = algorithmic bytes of the step / step time, in GB/s (16 N bytes per
+         complex transform, 8 N per real transform of N reals).
+Also reported: the plain per-call path (one FFT<float>::fft call per
+transform), the dominant kernel against the HBM roofline, rel L2 error
+against the reference's FFT<double>, and the reference CPU path on the host's
+cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "batched FP32 FFT GB/s (% HBM peak) vs plain per-call; serving p50/p99 latency"
+C2C_SIZES = (1920, 3000, 16384)
+C2C_BATCH = 8192
+REAL_N, REAL_FRAMES = 2048, 16384
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--plain-sample", type=int, default=256,
+                    help="transforms per size timed on the plain per-call path")
+    return ap.parse_args()
+
+
+class Launch:
+    """One batched launch of the step: a plan, buffers, a direction."""
+
+    def __init__(self, name, plan, x, y, batch, direction, alg_bytes):
+        self.name, self.plan, self.x, self.y = name, plan, x, y
+        self.batch, self.direction, self.alg_bytes = batch, direction, alg_bytes
+        self.times_ms = []
+
+    def run(self, stream_handle):
+        self.plan.execute_ptr(self.x.data_ptr(), self.y.data_ptr(), self.batch, self.direction,
+                              stream_handle)
+
+
+def build_workload(torch, fft_amd, L):
+    from fft_amd import synth
+    launches = []
+    bufs = {}
+    for n in C2C_SIZES:
+        plan = fft_amd.Plan(n, L.C2C, L.F32)
+        x = torch.empty(C2C_BATCH, n, dtype=torch.complex64, device="cuda")
+        fft_amd.fill(x, seed=n, dist="normal")
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        bufs[n] = (x, y, z)
+        ab = 16 * n * C2C_BATCH
+        launches.append(Launch(f"c2c_{n}_fwd", plan, x, y, C2C_BATCH, L.FORWARD, ab))
+        launches.append(Launch(f"c2c_{n}_inv", plan, y, z, C2C_BATCH, L.INVERSE, ab))
+    # RealFFT frames: tones from the seeded stream, noise generated in HBM
+    n, frames = REAL_N, REAL_FRAMES
+    fr = torch.from_numpy(synth.uniform01(n, frames * 8).reshape(frames, 8) * (n / 2)).cuda()
+    ph = torch.from_numpy(synth.uniform01(n ^ 0x5A5A5A5A, frames * 8).reshape(frames, 8)
+                          * (2 * math.pi)).cuda()
+    t = torch.arange(n, dtype=torch.float64, device="cuda")
+    xr = torch.zeros(frames, n, dtype=torch.float64, device="cuda")
+    for k in range(8):
+        xr += torch.cos(2 * math.pi * fr[:, k:k + 1] * t[None, :] / n + ph[:, k:k + 1])
+    noise = torch.empty(frames, n, dtype=torch.float32, device="cuda")
+    fft_amd.fill(noise, seed=n ^ 0xA5A5A5A5, dist="normal")
+    xr = (xr + 0.01 * noise.double()).float().contiguous()
+    del noise
+    yr = torch.empty(frames, n // 2, dtype=torch.complex64, device="cuda")
+    zr = torch.empty_like(xr)
+    bufs["real"] = (xr, yr, zr)
+    plan = fft_amd.Plan(n, L.R2C, L.F32)
+    ab = 8 * n * frames
+    launches.append(Launch(f"r2c_{n}_fwd", plan, xr, yr, frames, L.FORWARD, ab))
+    launches.append(Launch(f"c2r_{n}_inv", plan, yr, zr, frames, L.INVERSE, ab))
+    torch.cuda.synchronize()
+    return launches, bufs
+
+
+def time_steps(torch, launches, steps, warmup):
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    for _ in range(warmup):
+        for ln in launches:
+            ln.run(sh)
+    torch.cuda.synchronize()
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in launches] for _ in range(steps)]
+    begin, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    begin.record(stream)
+    for s in range(steps):
+        for i, ln in enumerate(launches):
+            ev[s][i][0].record(stream)
+            ln.run(sh)
+            ev[s][i][1].record(stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = begin.elapsed_time(end)
+    for s in range(steps):
+        for i, ln in enumerate(launches):
+            ln.times_ms.append(ev[s][i][0].elapsed_time(ev[s][i][1]))
+    return total_ms, wall
+
+
+def plain_per_call(torch, fft_amd, L, bufs, sample):
+    """One FFT<float>::fft call (one launch, batch 1) per transform, on the
+    same resident data; sample transforms per size and direction."""
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    total_bytes, total_ms = 0, 0.0
+    per = {}
+    for n in C2C_SIZES:
+        plan = fft_amd.Plan(n, L.C2C, L.F32)
+        x, y, _ = bufs[n]
+        for direction in (L.FORWARD, L.INVERSE):
+            for i in range(4):  # warm
+                plan.execute_ptr(x[i].data_ptr(), y[i].data_ptr(), 1, direction, sh)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for i in range(sample):
+                plan.execute_ptr(x[i].data_ptr(), y[i].data_ptr(), 1, direction, sh)
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b)
+            by = 16 * n * sample
+            per[f"c2c_{n}_{'fwd' if direction == L.FORWARD else 'inv'}"] = round(by / ms / 1e6, 2)
+            total_bytes += by
+            total_ms += ms
+    return total_bytes / total_ms / 1e6, per
+
+
+def cpu_baseline(torch, bufs, launches):
+    """The reference's own FFT<float> / RealFFT<float> on the host cores (one
+    transform per thread, each thread its own instance), over the whole step's
+    transforms once; plus rel L2 of sample GPU outputs against the reference's
+    FFT<double> / RealFFT<double>."""
+    from oracle import pyoracle as po
+    threads = min(16, os.cpu_count() or 1)
+    out = {"threads": threads}
+    errors = {}
+    sample_rows = np.linspace(0, C2C_BATCH - 1, 8).astype(int)
+    if po.ref_available():
+        kind = "reference"
+        total_bytes, total_s = 0, 0.0
+        for n in C2C_SIZES:
+            x, y, z = bufs[n]
+            xh = x.cpu().numpy()
+            yh = y.cpu().numpy()
+            for inverse, src in ((False, xh), (True, yh)):
+                total_s += po.ref_time_c2c(src, inverse, threads)
+                total_bytes += 16 * n * C2C_BATCH
+            ref64 = po.ref_fft(xh[sample_rows].astype(np.complex128))
+            errors[f"c2c_{n}_fwd"] = po.rel_l2(yh[sample_rows], ref64)
+            zr = z[sample_rows].cpu().numpy()
+            errors[f"c2c_{n}_inv"] = po.rel_l2(
+                zr, po.ref_fft(yh[sample_rows].astype(np.complex128), inverse=True))
+            del xh, yh
+        xr, yr, zr_ = bufs["real"]
+        xh = xr.cpu().numpy()
+        total_s += po.ref_time_real(xh, threads)
+        total_s += po.ref_time_real(yr.cpu().numpy(), threads, inverse=True)
+        total_bytes += 2 * 8 * REAL_N * REAL_FRAMES
+        rows = np.linspace(0, REAL_FRAMES - 1, 8).astype(int)
+        yh = yr[rows].cpu().numpy()
+        errors[f"r2c_{REAL_N}_fwd"] = po.rel_l2(yh, po.ref_real_fft(xh[rows].astype(np.float64)))
+        errors[f"c2r_{REAL_N}_inv"] = po.rel_l2(
+            zr_[rows].cpu().numpy(), po.ref_real_ifft(yh.astype(np.complex128)))
+        value = total_bytes / total_s / 1e9
+        out.update({"value": round(value, 3), "unit": "GB/s", "cores": threads, "kind": kind,
+                    "sample": "the whole step once: every transform of the step, forward and "
+                              "inverse (reference FFT<float>/RealFFT<float>, g++ -O3 "
+                              "-march=x86-64-v3)"})
+    else:
+        kind = "port"
+        x = bufs[1920][0][:64].cpu().numpy()
+        t0 = time.perf_counter()
+        po.fft(x.astype(np.complex128))
+        dt = time.perf_counter() - t0
+        out.update({"value": round(16 * 1920 * 64 / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
+                    "kind": kind, "sample": "oracle FFT<double> restatement, 64 x 1920"})
+    return out, errors
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    import fft_amd
+    from fft_amd import _lib as L
+
+    torch.cuda.set_device(0)
+    launches, bufs = build_workload(torch, fft_amd, L)
+    total_ms, wall = time_steps(torch, launches, args.steps, args.warmup)
+    step_bytes = sum(ln.alg_bytes for ln in launches)
+    ms_per_step = total_ms / args.steps
+    value = step_bytes / (ms_per_step * 1e-3) / 1e9
+
+    per = {}
+    for ln in launches:
+        avg = float(np.mean(ln.times_ms))
+        per[ln.name] = {"ms": round(avg, 4), "GB/s": round(ln.alg_bytes / avg / 1e6, 1),
+                        "hbm_frac": round(ln.alg_bytes / avg / 1e6 / HBM_PEAK_GBS, 4)}
+    dom = max(launches, key=lambda ln: float(np.sum(ln.times_ms)))
+    dom_ms = float(np.mean(dom.times_ms))
+    achieved = dom.alg_bytes / dom_ms / 1e6
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dom.name)
+
+    plain_gbs, plain_per = plain_per_call(torch, fft_amd, L, bufs, args.plain_sample)
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded splitmix64 N(0,1) complex64 (c2c), 8 sinusoids + 0.01 N(0,1) "
+                "frames (real), generated in HBM",
+        "config": {
+            "workload": "batched FFT<float> N in {1920, 3000, 16384} x 8192 fwd+inv and "
+                        "RealFFT<float> N=2048 x 16384 frames fwd+inv",
+            "c2c_sizes": list(C2C_SIZES), "c2c_batch": C2C_BATCH,
+            "real_n": REAL_N, "real_frames": REAL_FRAMES,
+            "bytes_per_step": step_bytes,
+        },
+        "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "per_launch": per,
+        "plain_per_call": {"value": round(plain_gbs, 2), "unit": "GB/s",
+                           "sample": f"{args.plain_sample} single-transform calls per size "
+                                     "and direction", "per_size": plain_per},
+        "speedup_vs_plain": round(value / plain_gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": dom.name, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "launch_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom.alg_bytes},
+        "wall_s_timed": round(wall, 3),
+    }
+    if not args.no_cpu:
+        cpu, errors = cpu_baseline(torch, bufs, launches)
+        result["cpu_baseline"] = cpu
+        result["rel_l2_vs_reference_fp64"] = {k: float(f"{v:.3e}") for k, v in errors.items()}
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -3,9 +3,10 @@
     FFT, RealFFT, ModifiedRealFFT   mirrors of signalsmith::FFT<V>, RealFFT<V>,
                                     ModifiedRealFFT<V> (fft_amd/api.py)
     Plan                            a cached engine plan (C ABI handle)
+    UnevenBatch                     many sizes in one persistent launch
     fill                            seeded synthetic inputs generated in HBM
 
 All transforms run through libsfft.so (include/sfft.h); importing this package
 does not load it -- the first call does, and raises if it is missing.
 """
-from .api import FFT, ModifiedRealFFT, Plan, RealFFT, fill  # noqa: F401
+from .api import FFT, ModifiedRealFFT, Plan, RealFFT, UnevenBatch, fill  # noqa: F401

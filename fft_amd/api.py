@@ -259,6 +259,69 @@ class ModifiedRealFFT(RealFFT):
     _half_shift = True
 
 
+def transform_bytes(n: int, kind: int, direction: int, precision: int):
+    """(input bytes, output bytes) of one transform."""
+    rb = 8 if precision == L.F64 else 4
+    if kind == L.C2C:
+        return 2 * n * rb, 2 * n * rb
+    if direction == L.FORWARD:
+        return n * rb, n * rb  # n reals -> n/2 complex
+    return n * rb, n * rb
+
+
+class UnevenBatch:
+    """Transforms of many sizes, kinds and directions in one persistent,
+    graph-captured launch (sfft_batch_*).  Inputs and outputs are packed back
+    to back, in the given order, in two device arenas."""
+
+    def __init__(self, sizes, directions, kinds=None, precision: int = L.F32):
+        m = len(sizes)
+        kinds = [L.C2C] * m if kinds is None else list(kinds)
+        items = (L.BatchItem * max(1, m))()
+        ioff = ooff = 0
+        self.in_offsets, self.out_offsets = [], []
+        for i, (n, d, k) in enumerate(zip(sizes, directions, kinds)):
+            ib, ob = transform_bytes(int(n), int(k), int(d), precision)
+            items[i] = L.BatchItem(int(n), int(k), int(d), 1, ioff, ooff)
+            self.in_offsets.append(ioff)
+            self.out_offsets.append(ooff)
+            ioff += ib
+            ooff += ob
+        self.in_bytes, self.out_bytes = ioff, ooff
+        self.sizes, self.directions, self.kinds = list(sizes), list(directions), kinds
+        self.precision = precision
+        h = ctypes.c_void_p()
+        L.check(L.lib().sfft_batch_create(ctypes.byref(h), items, m, precision))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and L._lib is not None:
+            L._lib.sfft_batch_destroy(h)
+            self._h = None
+
+    def _check(self, a: torch.Tensor, b: torch.Tensor):
+        if not (a.is_cuda and b.is_cuda):
+            raise ValueError("device arenas expected")
+        if a.numel() * a.element_size() < self.in_bytes or \
+                b.numel() * b.element_size() < self.out_bytes:
+            raise ValueError("arena too small")
+
+    def execute(self, in_arena: torch.Tensor, out_arena: torch.Tensor, stream=None):
+        self._check(in_arena, out_arena)
+        L.check(L.lib().sfft_batch_execute(self._h, ctypes.c_void_p(in_arena.data_ptr()),
+                                           ctypes.c_void_p(out_arena.data_ptr()),
+                                           ctypes.c_void_p(_stream_handle(stream))))
+        return out_arena
+
+    def execute_per_size(self, in_arena: torch.Tensor, out_arena: torch.Tensor, stream=None):
+        self._check(in_arena, out_arena)
+        L.check(L.lib().sfft_batch_execute_per_size(
+            self._h, ctypes.c_void_p(in_arena.data_ptr()), ctypes.c_void_p(out_arena.data_ptr()),
+            ctypes.c_void_p(_stream_handle(stream))))
+        return out_arena
+
+
 def fill(t: torch.Tensor, seed: int, dist: str = "uniform", stream=None) -> torch.Tensor:
     """Fill a device tensor with the seeded synthetic stream of fft_amd.synth
     (uniform [-1, 1) or normal), generated on the device."""

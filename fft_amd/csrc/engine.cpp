@@ -401,6 +401,39 @@ size_t Plan::out_bytes(size_t batch, int inverse) const {
 }
 
 template <typename T>
+PassArgs<T> Plan::pass_args(size_t p, int inverse) const {
+    const PassPlan &pp = passes_[p];
+    const size_t np = passes_.size();
+    const int64_t din = (int64_t)default_in_dist(inverse) / ((kind_ != kC2C && !inverse) ? 2 : 1);
+    const int64_t dout = (int64_t)default_out_dist(inverse) / ((kind_ != kC2C && inverse) ? 2 : 1);
+    PassArgs<T> a{};
+    a.pd = pp.pd;
+    const FastDiv pb = make_fastdiv((uint32_t)pp.slots_per_transform);
+    a.iin = IoDesc{pp.in_from_out ? dout : din, pp.in_slot_stride, pp.in_elem_stride, pb};
+    a.iout = IoDesc{dout, pp.out_slot_stride, pp.out_elem_stride, pb};
+    a.tw = (const C2<T> *)d_tw_;
+    a.conj_in = inverse && p == 0;
+    a.conj_out = inverse && p + 1 == np;
+    a.mode = kind_ == kC2C ? kModeC2C : (inverse ? kModeC2R : kModeR2C);
+    a.half_shift = kind_ == kR2CHalf;
+    a.post_tw = pp.post_tw;
+    a.post_tw_shift = ptw_shift_;
+    a.real_tw = (const C2<T> *)d_real_tw_;
+    a.rot = (const C2<T> *)d_rot_;
+    a.ptw_fine = d_ptw_fine_;
+    a.ptw_coarse = d_ptw_coarse_;
+    a.in_from_out = pp.in_from_out;
+    return a;
+}
+template PassArgs<float> Plan::pass_args<float>(size_t, int) const;
+template PassArgs<double> Plan::pass_args<double>(size_t, int) const;
+
+size_t Plan::slot_lds_bytes(size_t p) const {
+    const PassPlan &pp = passes_[p];
+    return pp.lds_bytes / (size_t)pp.pd.tpw;
+}
+
+template <typename T>
 void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
                int inverse, hipStream_t stream) const {
     // distances in complex elements
@@ -419,25 +452,12 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         const PassPlan &pp = passes_[p];
         const size_t slots = batch * (size_t)pp.slots_per_transform;
         if (slots > 0x7fffffffull) throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
-        PassArgs<T> a{};
-        a.pd = pp.pd;
-        const FastDiv pb = make_fastdiv((uint32_t)pp.slots_per_transform);
-        a.iin = IoDesc{pp.in_from_out ? dout : din, pp.in_slot_stride, pp.in_elem_stride, pb};
-        a.iout = IoDesc{dout, pp.out_slot_stride, pp.out_elem_stride, pb};
+        PassArgs<T> a = pass_args<T>(p, inverse);
+        a.iin.batch_stride = pp.in_from_out ? dout : din;
+        a.iout.batch_stride = dout;
         a.in = pp.in_from_out ? (const C2<T> *)out : (const C2<T> *)in;
         a.out = (C2<T> *)out;
-        a.tw = (const C2<T> *)d_tw_;
         a.total_slots = (int32_t)slots;
-        a.conj_in = inverse && p == 0;
-        a.conj_out = inverse && p + 1 == np;
-        a.mode = kind_ == kC2C ? kModeC2C : (inverse ? kModeC2R : kModeR2C);
-        a.half_shift = kind_ == kR2CHalf;
-        a.post_tw = pp.post_tw;
-        a.post_tw_shift = ptw_shift_;
-        a.real_tw = (const C2<T> *)d_real_tw_;
-        a.rot = (const C2<T> *)d_rot_;
-        a.ptw_fine = d_ptw_fine_;
-        a.ptw_coarse = d_ptw_coarse_;
         const unsigned grid = (unsigned)((slots + pp.pd.tpw - 1) / pp.pd.tpw);
         check_hip(launch_pass<T>(a, pp.ept, grid, pp.block, pp.lds_bytes, stream), "pass launch");
     }

@@ -80,6 +80,13 @@ class Plan {
     void execute(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
                  int inverse, hipStream_t stream) const;
 
+    // Static launch arguments of pass p for a direction (in/out/total_slots
+    // left empty; batch distances = contiguous defaults).
+    template <typename T>
+    PassArgs<T> pass_args(size_t p, int inverse) const;
+    // LDS bytes one slot of pass p needs (0 if the pass never touches LDS).
+    size_t slot_lds_bytes(size_t p) const;
+
     // Default distances for a contiguous batch.
     size_t default_in_dist(int inverse) const;
     size_t default_out_dist(int inverse) const;

@@ -10,7 +10,8 @@ template <typename T, int EPT>
 __global__ void __launch_bounds__(1024) k_pass(PassArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
-    Block<T, EPT> blk(a, lds, (int)threadIdx.x, (int)blockIdx.x * a.pd.tpw);
+    Block<T, EPT> blk(a, a.in, a.out, a.total_slots, lds, (int)threadIdx.x,
+                      (int)blockIdx.x * a.pd.tpw);
     blk.run();
 }
 
@@ -19,7 +20,7 @@ static hipError_t launch_one(const PassArgs<T> &a, unsigned grid, unsigned block
                              hipStream_t stream) {
     static bool attr = false;
     if (!attr) {
-        hipFuncSetAttribute((const void *)k_pass<T, EPT>,
+        (void)hipFuncSetAttribute((const void *)k_pass<T, EPT>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }

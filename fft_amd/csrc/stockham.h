@@ -44,17 +44,36 @@ struct PassArgs {
     const C2<T> *rot;      // half-shift pre-rotations e^{-2 pi i i / N}, N/2 entries
     const C2<double> *ptw_fine;    // W_N^{m}, m < 2^shift
     const C2<double> *ptw_coarse;  // W_N^{m 2^shift}
+    int32_t in_from_out;   // second pass of a two-pass plan: reads the output buffer
+    int32_t pad_;
 };
 
+// The pass descriptor `a` may live in kernel arguments or in a device table
+// (persistent launches); in, out and total_slots are per launch / per item.
+// Optional gather lists (persistent launches) give each slot its own input /
+// output offset in elements instead of the strided address map.
 template <typename T, int EPT>
 struct Block {
     const PassArgs<T> &a;
+    const C2<T> *in;
+    C2<T> *out;
+    int total_slots;
     C2<T> *lds;
     int tid;
     int slot0;
+    const int64_t *gather_in = nullptr;
+    const int64_t *gather_out = nullptr;
 
-    __device__ Block(const PassArgs<T> &args, C2<T> *l, int t, int s0)
-        : a(args), lds(l), tid(t), slot0(s0) {}
+    __device__ Block(const PassArgs<T> &args, const C2<T> *in_, C2<T> *out_, int total, C2<T> *l,
+                     int t, int s0)
+        : a(args), in(in_), out(out_), total_slots(total), lds(l), tid(t), slot0(s0) {}
+
+    __device__ __forceinline__ int64_t ibase(int g) const {
+        return gather_in ? gather_in[g] : gbase(a.iin, g);
+    }
+    __device__ __forceinline__ int64_t obase(int g) const {
+        return gather_out ? gather_out[g] : gbase(a.iout, g);
+    }
 
     __device__ __forceinline__ int64_t gbase(const IoDesc &io, int g) const {
         if (io.per_batch.d <= 1) return (int64_t)g * io.batch_stride;
@@ -90,13 +109,13 @@ struct Block {
             lid = tid % pd.tpt;
         }
         const int g = slot0 + slot;
-        const bool active = (slot < pd.tpw) && (g < a.total_slots);
+        const bool active = (slot < pd.tpw) && (lid < pd.tpt) && (g < total_slots);
         C2<T> *ls = lds + slot * pd.lds_stride;
 
         C2<T> v[B][R];
         // ---- load
         if (src_global) {
-            const int64_t base = active ? gbase(a.iin, g) : 0;
+            const int64_t base = active ? ibase(g) : 0;
             const int64_t es = a.iin.elem_stride;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -105,7 +124,7 @@ struct Block {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int e = j + r * nb;
-                        C2<T> x = a.in[base + (int64_t)e * es];
+                        C2<T> x = in[base + (int64_t)e * es];
                         if (a.conj_in) x.y = -x.y;
                         if (a.mode == kModeR2C && a.half_shift) x = cmul(x, a.rot[e]);
                         v[b][r] = x;
@@ -141,7 +160,7 @@ struct Block {
         }
         // ---- store
         if (dst_global) {
-            const int64_t base = active ? gbase(a.iout, g) : 0;
+            const int64_t base = active ? obase(g) : 0;
             const int64_t es = a.iout.elem_stride;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -157,7 +176,7 @@ struct Block {
                         if (a.conj_out) y.y = -y.y;
                         if (a.mode == kModeC2R && a.half_shift) y = cmulc(y, a.rot[e]);
                         if (a.post_tw) y = cmul(y, post_twiddle(g, e));
-                        a.out[base + (int64_t)e * es] = y;
+                        out[base + (int64_t)e * es] = y;
                     }
                 }
             }
@@ -204,12 +223,12 @@ struct Block {
         const PassDesc &pd = a.pd;
         const int slot = tid / pd.tpt, lid = tid % pd.tpt;
         const int g = slot0 + slot;
-        const bool active = (slot < pd.tpw) && (g < a.total_slots);
+        const bool active = (slot < pd.tpw) && (g < total_slots);
         const int h = pd.n;
         const bool mod = a.half_shift != 0;
         C2<T> *ls = lds + slot * pd.lds_stride;
         if (active) {
-            const C2<T> *x = a.in + gbase(a.iin, g);
+            const C2<T> *x = in + ibase(g);
             if (!mod && lid == 0) {
                 const C2<T> v = x[0];
                 ls[0] = {v.x + v.y, -(v.x - v.y)};
@@ -234,12 +253,12 @@ struct Block {
         const PassDesc &pd = a.pd;
         const int slot = tid / pd.tpt, lid = tid % pd.tpt;
         const int g = slot0 + slot;
-        const bool active = (slot < pd.tpw) && (g < a.total_slots);
+        const bool active = (slot < pd.tpw) && (g < total_slots);
         const int h = pd.n;
         const bool mod = a.half_shift != 0;
         const C2<T> *ls = lds + slot * pd.lds_stride;
         if (!active) return;
-        C2<T> *y = a.out + gbase(a.iout, g);
+        C2<T> *y = out + obase(g);
         if (!mod && lid == 0) {
             const C2<T> z = ls[0];
             y[0] = {z.x + z.y, z.x - z.y};

@@ -233,13 +233,22 @@ def ref_time_c2c(x: np.ndarray, inverse: bool, threads: int) -> float:
                                 int(inverse), threads)
 
 
-def ref_time_real(x: np.ndarray, threads: int) -> float:
+def ref_time_real(x: np.ndarray, threads: int, inverse: bool = False) -> float:
+    """Wall seconds for RealFFT<float> over every row: forward takes (..., n)
+    reals, inverse takes (..., n/2) complex."""
     lib = _load_ref()
-    x = np.ascontiguousarray(x, dtype=np.float32)
-    n = x.shape[-1]
-    batch = x.size // n
-    out = np.empty(x.shape[:-1] + (n // 2,), dtype=np.complex64)
-    return lib.ref_time_real_f32(n, batch, x.ctypes.data_as(_fp), out.ctypes.data_as(_fp), 0, threads)
+    if inverse:
+        x = np.ascontiguousarray(x, dtype=np.complex64)
+        n = 2 * x.shape[-1]
+        batch = x.size // (n // 2)
+        out = np.empty(x.shape[:-1] + (n,), dtype=np.float32)
+    else:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = x.shape[-1]
+        batch = x.size // n
+        out = np.empty(x.shape[:-1] + (n // 2,), dtype=np.complex64)
+    return lib.ref_time_real_f32(n, batch, x.ctypes.data_as(_fp), out.ctypes.data_as(_fp),
+                                 int(inverse), threads)
 
 
 # ---------------------------------------------------------------- metrics --

@@ -1,0 +1,109 @@
+"""Uneven batches in one persistent, graph-captured launch.
+
+Each transform's plan depends only on (size, kind, precision), so the result
+of a transform is bitwise the same alone, in the per-size loop and in the
+persistent mix; the oracle pins a sample to rel L2 1e-6.
+"""
+import numpy as np
+import pytest
+import torch
+
+import fft_amd
+from fft_amd import _lib as L
+from fft_amd import workloads
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-6
+
+
+def _view_c64(arena, off, n):
+    return arena[off:off + 8 * n].view(torch.complex64)
+
+
+def _make(sizes, dirs, kinds=None, seed=11):
+    b = fft_amd.UnevenBatch([int(s) for s in sizes], [int(d) for d in dirs], kinds)
+    xin = torch.empty(b.in_bytes, dtype=torch.uint8, device="cuda")
+    fft_amd.fill(xin.view(torch.float32), seed=seed, dist="uniform")
+    return b, xin
+
+
+def test_uneven_batch_matches_per_size_and_oracle():
+    sizes, dirs = workloads.uneven_batch(seed=3, count=3000)
+    b, xin = _make(sizes, dirs)
+    out_p = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    out_s = torch.zeros_like(out_p)
+    b.execute(xin, out_p)
+    b.execute_per_size(xin, out_s)
+    torch.cuda.synchronize()
+    assert torch.equal(out_p, out_s), "persistent mix differs from the per-size loop"
+    rng = np.random.default_rng(0)
+    for i in rng.choice(len(sizes), 40, replace=False):
+        n, d = int(sizes[i]), int(dirs[i])
+        x = _view_c64(xin, b.in_offsets[i], n).cpu().numpy()
+        y = _view_c64(out_p, b.out_offsets[i], n).cpu().numpy()
+        err = po.rel_l2(y, po.fft(x.astype(np.complex128), inverse=bool(d)))
+        assert err <= TOL32, (n, d, err)
+
+
+def test_same_result_alone_and_in_the_mix():
+    sizes, dirs = workloads.uneven_batch(seed=5, count=600)
+    b, xin = _make(sizes, dirs, seed=5)
+    out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    b.execute(xin, out)
+    torch.cuda.synchronize()
+    for i in range(0, 600, 37):
+        n, d = int(sizes[i]), int(dirs[i])
+        x = _view_c64(xin, b.in_offsets[i], n).clone()[None, :]
+        y = torch.empty_like(x)
+        f = fft_amd.FFT(n)
+        (f.ifft if d else f.fft)(x, y)
+        torch.cuda.synchronize()
+        assert torch.equal(y[0], _view_c64(out, b.out_offsets[i], n)), (i, n, d)
+
+
+def test_graph_replays_are_stable():
+    sizes, dirs = workloads.uneven_batch(seed=7, count=800)
+    b, xin = _make(sizes, dirs, seed=7)
+    ref = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    b.execute_per_size(xin, ref)
+    out = torch.empty_like(ref)
+    for r in range(4):
+        out.zero_()
+        b.execute(xin, out)  # graph replay (captured on the first call)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), f"replay {r} differs"
+        assert torch.isfinite(junk)
+
+
+def test_mixed_kinds_in_one_launch():
+    sizes = [2048, 4096, 1920, 3000, 256, 65536, 2048, 30000, 100, 6]
+    kinds = [L.R2C, L.C2C, L.R2C_HALFSHIFT, L.C2C, L.R2C, L.C2C, L.R2C, L.C2C, L.R2C_HALFSHIFT,
+             L.R2C]
+    dirs = [0, 1, 1, 0, 1, 0, 1, 1, 0, 0]
+    b, xin = _make(sizes, dirs, kinds, seed=13)
+    out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    ref = torch.empty_like(out)
+    b.execute(xin, out)
+    b.execute_per_size(xin, ref)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    for i, (n, k, d) in enumerate(zip(sizes, kinds, dirs)):
+        hs = k == L.R2C_HALFSHIFT
+        ib = xin[b.in_offsets[i]:b.in_offsets[i] + 8 * n]
+        ob = out[b.out_offsets[i]:b.out_offsets[i] + 8 * n]
+        if k == L.C2C:
+            x = ib.view(torch.complex64).cpu().numpy()
+            y = ob.view(torch.complex64).cpu().numpy()
+            want = po.fft(x.astype(np.complex128), inverse=bool(d))
+        elif d == 0:
+            x = ib[: 4 * n].view(torch.float32).cpu().numpy()
+            y = ob[: 4 * n].view(torch.complex64).cpu().numpy()
+            want = po.real_fft(x.astype(np.float64), half_shift=hs)
+        else:
+            x = ib[: 4 * n].view(torch.complex64).cpu().numpy()
+            y = ob[: 4 * n].view(torch.float32).cpu().numpy()
+            want = po.real_ifft(x.astype(np.complex128), half_shift=hs)
+        assert po.rel_l2(y, want) <= TOL32, (n, k, d)

@@ -168,7 +168,7 @@ def test_batched_config_sizes(n):
     zi = _run(n, y, inverse=True)
     assert po.rel_l2(zi, po.ifft(y.astype(np.complex128))) <= TOL32
     # whole batch: round trip = N x
-    rt = (zd.double() / n - xd.to(torch.complex128)).abs().pow(2).sum().sqrt() / \
+    rt = (zd.to(torch.complex128) / n - xd.to(torch.complex128)).abs().pow(2).sum().sqrt() / \
         xd.to(torch.complex128).abs().pow(2).sum().sqrt()
     assert rt.item() <= 2 * TOL32, rt.item()
     # a row alone equals the row in the batch, bitwise
