@@ -15,7 +15,7 @@ UnevenBatch::UnevenBatch(const sfft_batch_item *items, size_t count, int prec)
             throw Error(SFFT_ERR_INVALID_ARG, "batch item direction");
         plans_.push_back(get_plan(it.n, it.kind, prec));
     }
-    const size_t lds_cap = device_info().lds_bytes;
+    const size_t lds_cap = device_info().lds_bytes - kLdsReserve;
     if (prec == kF64)
         build<double>(lds_cap);
     else
@@ -131,18 +131,20 @@ void UnevenBatch::build(size_t lds_cap) {
 }
 
 void UnevenBatch::launch(const void *in_arena, void *out_arena, hipStream_t stream) {
+    const std::string what = "persistent launch (grid " + std::to_string(grid_) + ", LDS " +
+                             std::to_string(lds_bytes_) + " B)";
     if (prec_ == kF64)
         check_hip(launch_persistent<double>(d_units_, (int)nunits_,
                                             (const PassArgs<double> *)d_table_, d_gather_in_,
                                             d_gather_out_, d_counters_,
                                             in_arena, out_arena, grid_, lds_bytes_, stream),
-                  "persistent launch");
+                  what.c_str());
     else
         check_hip(launch_persistent<float>(d_units_, (int)nunits_,
                                            (const PassArgs<float> *)d_table_, d_gather_in_,
                                            d_gather_out_, d_counters_,
                                            in_arena, out_arena, grid_, lds_bytes_, stream),
-                  "persistent launch");
+                  what.c_str());
 }
 
 void UnevenBatch::execute_persistent(const void *in_arena, void *out_arena, hipStream_t stream) {

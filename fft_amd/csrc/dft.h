@@ -20,18 +20,23 @@ struct alignas(2 * sizeof(T)) C2 {
     T x, y;
 };
 
+// Fused multiply-add spelled out: the library builds with -ffp-contract=off,
+// so every kernel that runs the same radices computes bit-identical results.
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
 __device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>
 __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    return {fmadd(a.x, b.x, -(a.y * b.y)), fmadd(a.x, b.y, a.y * b.x)};
 }
 // a * conj(b)
 template <typename T>
 __device__ __forceinline__ C2<T> cmulc(C2<T> a, C2<T> b) {
-    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+    return {fmadd(a.x, b.x, a.y * b.y), fmadd(a.y, b.x, -(a.x * b.y))};
 }
 template <typename T>
 __device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return {a.x * s, a.y * s}; }
@@ -129,7 +134,7 @@ __device__ __forceinline__ C2<T> twc(C2<T> x) {
         constexpr ct::CS w = ct::cossin(e, R);
         constexpr T c = (T)w.c;
         constexpr T s = (T)(-w.s);
-        return {x.x * c - x.y * s, x.x * s + x.y * c};
+        return {fmadd(x.x, c, -(x.y * s)), fmadd(x.x, s, x.y * c)};
     }
 }
 
@@ -164,7 +169,7 @@ __device__ __forceinline__ void dft_base3(C2<T> (&v)[3]) {
     constexpr T c = (T)-0.5;
     constexpr T s = (T)0.86602540378443864676372317075293618;
     C2<T> t1 = cadd(v[1], v[2]);
-    C2<T> t2 = {v[0].x + c * t1.x, v[0].y + c * t1.y};
+    C2<T> t2 = {fmadd(c, t1.x, v[0].x), fmadd(c, t1.y, v[0].y)};
     C2<T> t3 = {s * (v[1].x - v[2].x), s * (v[1].y - v[2].y)};
     v[0] = cadd(v[0], t1);
     v[1] = {t2.x + t3.y, t2.y - t3.x};  // t2 - i t3
@@ -202,10 +207,10 @@ __device__ __forceinline__ void dft_odd(C2<T> (&v)[R]) {
             constexpr int m = decltype(mi)::value + 1;
             constexpr ct::CS w = ct::cossin((long long)m * k, R);
             constexpr T c = (T)w.c, s = (T)w.s;
-            t.x += c * a[m - 1].x;
-            t.y += c * a[m - 1].y;
-            u.x += s * b[m - 1].x;
-            u.y += s * b[m - 1].y;
+            t.x = fmadd(c, a[m - 1].x, t.x);
+            t.y = fmadd(c, a[m - 1].y, t.y);
+            u.x = fmadd(s, b[m - 1].x, u.x);
+            u.y = fmadd(s, b[m - 1].y, u.y);
         });
         v[k] = {t.x + u.y, t.y - u.x};      // t - i u
         v[R - k] = {t.x - u.y, t.y + u.x};  // t + i u

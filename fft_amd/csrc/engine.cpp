@@ -66,7 +66,7 @@ size_t real_size_maximum(size_t n) { return size_minimum(n / 2) * 2; }
 
 // --------------------------------------------------------------- planning --
 
-static const int kRadixSet[] = {16, 15, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+static const int kRadixSet[] = {25, 16, 15, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
 
 static int butterflies_per_thread(int radix, int ept) { return std::max(1, ept / radix); }
 
@@ -91,6 +91,9 @@ static double radix_cost(int n, const std::vector<int> &radices, int ept, int tp
 
 static void search_radices(int rem, int max_r, std::vector<int> &cur, int n, int ept,
                            double &best_cost, std::vector<int> &best) {
+    // 25 complex values exceed the register budget of the 16-element instance
+    // in FP64 and of small per-thread budgets: only offered when ept >= 16 and
+    // the caller allows it (max_r).
     if (rem == 1) {
         const int tpt = threads_for(n, cur, ept);
         if (tpt > 1024) return;
@@ -110,7 +113,7 @@ static void search_radices(int rem, int max_r, std::vector<int> &cur, int n, int
     }
 }
 
-bool choose_radices(int n, int ept, std::vector<int> &radices) {
+bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix) {
     radices.clear();
     if (n < 1) return false;
     if (n == 1) {
@@ -119,7 +122,7 @@ bool choose_radices(int n, int ept, std::vector<int> &radices) {
     }
     std::vector<int> cur;
     double best_cost = 1e30;
-    search_radices(n, 1 << 30, cur, n, ept, best_cost, radices);
+    search_radices(n, max_radix, cur, n, ept, best_cost, radices);
     return !radices.empty();
 }
 
@@ -132,6 +135,10 @@ static bool only_small_primes(size_t n) {
 bool supported_length(size_t n) { return n >= 1 && n <= (1ull << 28) && only_small_primes(n); }
 
 static size_t elem_bytes(int prec) { return prec == kF64 ? 16 : 8; }
+
+// Dynamic LDS a layout may use: the device limit less room for the kernels'
+// static __shared__ variables.
+static size_t lds_budget() { return device_info().lds_bytes - kLdsReserve; }
 
 static int floor_pow2(int x) {
     int p = 1;
@@ -156,7 +163,6 @@ static void fill_stages(PassDesc &pd, int n, const std::vector<int> &radices) {
 // Single-pass layout for sub-transform length n; false if it does not fit.
 static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints *h,
                           PassPlan &pp) {
-    const DeviceInfo &dev = device_info();
     const size_t eb = elem_bytes(prec);
     std::vector<int> radices;
     int ept = 0;
@@ -170,7 +176,7 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
             if (prod != n) return false;
             radices = h->radices;
             if (threads_for(n, radices, e) > 1024) continue;
-        } else if (!choose_radices(n, e, radices)) {
+        } else if (!choose_radices(n, e, radices, prec == kF64 ? 16 : 25)) {
             continue;
         }
         ept = e;
@@ -181,11 +187,11 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
     const int stride = lds_padded(n);
     const bool uses_lds = uses_lds_always || radices.size() > 1;
     const size_t slot_lds = uses_lds ? (size_t)stride * eb : 0;
-    if (slot_lds > dev.lds_bytes) return false;
+    if (slot_lds > lds_budget()) return false;
     int tpw = std::max(1, 256 / tpt);
     if (h && h->tpw) tpw = h->tpw;
     if (tpw * tpt > 1024) tpw = std::max(1, 1024 / tpt);
-    if (slot_lds) tpw = std::min<int>(tpw, (int)(dev.lds_bytes / slot_lds));
+    if (slot_lds) tpw = std::min<int>(tpw, (int)(lds_budget() / slot_lds));
     if (tpw < 1) return false;
     pp = PassPlan{};
     fill_stages(pp.pd, n, radices);
@@ -201,13 +207,12 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
 // Column-pass layout: `cols` adjacent slots per block along the contiguous
 // dimension (coalesced column access); cols is capped by LDS and 1024 threads.
 static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, PassPlan &pp) {
-    const DeviceInfo &dev = device_info();
     const size_t eb = elem_bytes(prec);
     std::vector<int> radices;
     int ept = 0;
     for (int e : {kEptSmall, kEptLarge}) {
         if (prec == kF64 && e > kEptSmall) continue;
-        if (choose_radices(n, e, radices)) {
+        if (choose_radices(n, e, radices, prec == kF64 ? 16 : 25)) {
             ept = e;
             break;
         }
@@ -218,7 +223,7 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     const int stride = lds_padded(n);
     const size_t slot_lds = (size_t)stride * eb;
     int cols = std::min(max_cols, 16);
-    cols = std::min<int>(cols, (int)(dev.lds_bytes / slot_lds));
+    cols = std::min<int>(cols, (int)(lds_budget() / slot_lds));
     cols = std::min(cols, 1024 / tpt);
     if (cols < 1) return false;
     cols = floor_pow2(cols);
@@ -258,7 +263,28 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     const bool force_two = h && h->split > 1;
     const bool force_one = h && h->split == 1;
     bool ok = false;
-    if (!force_two && cn_ <= (1u << 20)) ok = layout_single((int)cn_, prec, kind != kC2C, h, single);
+    // A compile-time planned kernel fixes the radices of this length.
+    const FixedKernel *fk = (h && !h->radices.empty())
+                                ? nullptr
+                                : find_fixed((int)cn_, prec, kind != kC2C ? 1 : 0);
+    if (fk && !force_two) {
+        PlanHints fh;
+        if (h) fh = *h;
+        fh.radices.assign(fk->radices, fk->radices + fk->nst);
+        ok = layout_single((int)cn_, prec, kind != kC2C, &fh, single);
+        if (ok) {
+            single.fixed = fk;
+            const int cus = device_info().cu_count;
+            if (kind == kC2C) {
+                single.fixed_grid[kModeC2C] = (unsigned)(fk->blocks_per_cu(kModeC2C) * cus);
+            } else {
+                single.fixed_grid[kModeR2C] = (unsigned)(fk->blocks_per_cu(kModeR2C) * cus);
+                single.fixed_grid[kModeC2R] = (unsigned)(fk->blocks_per_cu(kModeC2R) * cus);
+            }
+        }
+    }
+    if (!ok && !force_two && cn_ <= (1u << 20))
+        ok = layout_single((int)cn_, prec, kind != kC2C, h, single);
     if (ok) {
         passes_.push_back(single);
     } else {
@@ -436,6 +462,11 @@ size_t Plan::slot_lds_bytes(size_t p) const {
 template <typename T>
 void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
                int inverse, hipStream_t stream) const {
+    const PassPlan &p0 = passes_[0];
+    if (p0.fixed && in_dist == default_in_dist(inverse) && out_dist == default_out_dist(inverse)) {
+        run_fixed<T>(in, out, batch, inverse, stream);
+        return;
+    }
     // distances in complex elements
     int64_t din = (int64_t)in_dist, dout = (int64_t)out_dist;
     if (kind_ != kC2C) {
@@ -462,6 +493,39 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         check_hip(launch_pass<T>(a, pp.ept, grid, pp.block, pp.lds_bytes, stream), "pass launch");
     }
 }
+
+// Compile-time planned kernel over a contiguous batch.  It addresses HBM
+// with 32-bit byte offsets from the launch base, so the batch is split into
+// launches spanning less than 4 GiB.
+template <typename T>
+void Plan::run_fixed(const void *in, void *out, size_t batch, int inverse,
+                     hipStream_t stream) const {
+    const PassPlan &pp = passes_[0];
+    const FixedKernel &fk = *pp.fixed;
+    const int mode = kind_ == kC2C ? kModeC2C : (inverse ? kModeC2R : kModeR2C);
+    const size_t eb = sizeof(C2<T>);
+    const size_t per = cn_ * eb;  // bytes per transform (input and output)
+    const size_t chunk = std::max<size_t>(1, ((1ull << 32) - 1) / per);
+    const uint32_t flip = inverse ? 0x80000000u : 0u;
+    for (size_t c0 = 0; c0 < batch; c0 += chunk) {
+        const size_t cb = std::min(chunk, batch - c0);
+        FixedArgs<T> a{};
+        a.in = (const C2<T> *)in + c0 * cn_;
+        a.out = (C2<T> *)out + c0 * cn_;
+        a.tw = (const C2<T> *)d_tw_;
+        a.real_tw = (const C2<T> *)d_real_tw_;
+        a.rot = (const C2<T> *)d_rot_;
+        a.total_slots = (int32_t)cb;
+        a.flip_in = mode == kModeC2C ? flip : 0u;
+        a.flip_out = flip;
+        a.half_shift = kind_ == kR2CHalf;
+        const size_t groups = (cb + fk.tpw - 1) / fk.tpw;
+        const unsigned grid = (unsigned)std::min<size_t>(groups, pp.fixed_grid[mode]);
+        check_hip(fk.launch(&a, mode, grid, stream), "fixed-plan launch");
+    }
+}
+template void Plan::run_fixed<float>(const void *, void *, size_t, int, hipStream_t) const;
+template void Plan::run_fixed<double>(const void *, void *, size_t, int, hipStream_t) const;
 
 void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
                    int inverse, hipStream_t stream) const {

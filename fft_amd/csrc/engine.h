@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "fixed.h"
 #include "kernels.h"
 #include "plan_desc.h"
 
@@ -51,6 +52,9 @@ struct PassPlan {
     int64_t out_slot_stride = 0, out_elem_stride = 1;
     unsigned block = 0;
     size_t lds_bytes = 0;
+    // compile-time planned kernel with the same radices (contiguous batches)
+    const FixedKernel *fixed = nullptr;
+    unsigned fixed_grid[3] = {0, 0, 0};  // resident blocks per mode (C2C, R2C, C2R)
 };
 
 // Overrides the planner's choice (autotuner candidates).
@@ -99,6 +103,8 @@ class Plan {
     template <typename T>
     void run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch, int inverse,
              hipStream_t stream) const;
+    template <typename T>
+    void run_fixed(const void *in, void *out, size_t batch, int inverse, hipStream_t stream) const;
 
     size_t n_;
     int kind_, prec_;
@@ -119,7 +125,7 @@ class Plan {
 std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec);
 
 // Radix choice for a single pass of length n (exposed for tests/tuning).
-bool choose_radices(int n, int ept, std::vector<int> &radices);
+bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix = 25);
 bool supported_length(size_t n);
 
 void check_hip(hipError_t e, const char *what);

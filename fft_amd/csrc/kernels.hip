@@ -20,8 +20,13 @@ static hipError_t launch_one(const PassArgs<T> &a, unsigned grid, unsigned block
                              hipStream_t stream) {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_pass<T, EPT>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass<T, EPT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024 - kLdsReserve);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return e;
+        }
         attr = true;
     }
     hipLaunchKernelGGL((k_pass<T, EPT>), dim3(grid), dim3(block), lds_bytes, stream, a);

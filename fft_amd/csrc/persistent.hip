@@ -67,8 +67,13 @@ hipError_t launch_persistent(const WorkUnit *units, int nunits, const PassArgs<T
     constexpr int E = persistent_ept<T>();
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void *)k_persistent<T, E>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipError_t e = hipFuncSetAttribute((const void *)k_persistent<T, E>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024 - kLdsReserve);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return e;
+        }
         attr = true;
     }
     hipLaunchKernelGGL((k_persistent<T, E>), dim3(grid), dim3(kPersistentBlock), lds_bytes,

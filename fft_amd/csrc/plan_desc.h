@@ -18,6 +18,7 @@ namespace sfft {
 
 constexpr int kMaxStages = 14;
 constexpr int kLdsPadShift = 4;  // one pad element per 16 (breaks power-of-two strides)
+constexpr int kLdsReserve = 256; // bytes of LDS kept for static __shared__ variables
 
 // Unsigned division by an invariant divisor: q = (umulhi(n, mul) + n) >> shift.
 // Exact for n < 2^31 (Granlund-Montgomery round-up method).

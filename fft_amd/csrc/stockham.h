@@ -89,7 +89,7 @@ struct Block {
         const uint32_t m = c * (uint32_t)e;
         const C2<double> f = a.ptw_fine[m & ((1u << a.post_tw_shift) - 1u)];
         const C2<double> h = a.ptw_coarse[m >> a.post_tw_shift];
-        return {(T)(f.x * h.x - f.y * h.y), (T)(f.x * h.y + f.y * h.x)};
+        return {(T)fmadd(f.x, h.x, -(f.y * h.y)), (T)fmadd(f.x, h.y, f.y * h.x)};
     }
 
     // One Stockham stage of radix R.
@@ -213,6 +213,7 @@ struct Block {
         case 13: stage<13>(s, src_global, dst_global, col_map); break;
         case 15: stage<15>(s, src_global, dst_global, col_map); break;
         case 16: stage<16>(s, src_global, dst_global, col_map); break;
+        case 25: stage<25>(s, src_global, dst_global, col_map); break;
         default: break;
         }
     }
