@@ -34,6 +34,8 @@ EXPORTED = [
     "sfft_execute", "sfft_execute_strided",
     "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_execute",
     "sfft_batch_execute_per_size",
+    "sfft_malloc_device", "sfft_free_device", "sfft_malloc_host", "sfft_free_host",
+    "sfft_memcpy", "sfft_stream_synchronize",
     "sfft_fill",
 ]
 

@@ -169,6 +169,65 @@ int sfft_batch_execute_per_size(sfft_batch batch, const void *in_arena, void *ou
     });
 }
 
+int sfft_malloc_device(void **ptr, size_t bytes) {
+    return guarded([&] {
+        if (!ptr) return fail(SFFT_ERR_INVALID_ARG, "NULL pointer");
+        *ptr = nullptr;
+        if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SFFT_ERR_OUT_OF_MEMORY, "device allocation failed");
+        }
+        return SFFT_OK;
+    });
+}
+
+int sfft_free_device(void *ptr) {
+    return guarded([&] {
+        if (ptr) sfft::check_hip(hipFree(ptr), "hipFree");
+        return SFFT_OK;
+    });
+}
+
+int sfft_malloc_host(void **ptr, size_t bytes) {
+    return guarded([&] {
+        if (!ptr) return fail(SFFT_ERR_INVALID_ARG, "NULL pointer");
+        *ptr = nullptr;
+        if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SFFT_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+        }
+        return SFFT_OK;
+    });
+}
+
+int sfft_free_host(void *ptr) {
+    return guarded([&] {
+        if (ptr) sfft::check_hip(hipHostFree(ptr), "hipHostFree");
+        return SFFT_OK;
+    });
+}
+
+int sfft_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+    return guarded([&] {
+        hipMemcpyKind k;
+        switch (kind) {
+        case SFFT_COPY_H2D: k = hipMemcpyHostToDevice; break;
+        case SFFT_COPY_D2H: k = hipMemcpyDeviceToHost; break;
+        case SFFT_COPY_D2D: k = hipMemcpyDeviceToDevice; break;
+        default: return fail(SFFT_ERR_INVALID_ARG, "copy kind");
+        }
+        if (bytes) sfft::check_hip(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream), "copy");
+        return SFFT_OK;
+    });
+}
+
+int sfft_stream_synchronize(void *stream) {
+    return guarded([&] {
+        sfft::check_hip(hipStreamSynchronize((hipStream_t)stream), "stream synchronize");
+        return SFFT_OK;
+    });
+}
+
 int sfft_fill(void *dst, size_t count, int precision, int dist, uint64_t seed, void *stream) {
     return guarded([&] {
         if (!dst && count) return fail(SFFT_ERR_INVALID_ARG, "NULL buffer");

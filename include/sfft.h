@@ -143,6 +143,21 @@ int sfft_batch_execute(sfft_batch batch, const void *in_arena, void *out_arena, 
 int sfft_batch_execute_per_size(sfft_batch batch, const void *in_arena, void *out_arena,
                                 void *stream);
 
+/* ---- memory helpers for host-side bindings (the drop-in header uses them to
+ * stage host iterators; no counterpart in the reference, which is host-only) ---- */
+
+#define SFFT_COPY_H2D 1
+#define SFFT_COPY_D2H 2
+#define SFFT_COPY_D2D 3
+
+int sfft_malloc_device(void **ptr, size_t bytes);
+int sfft_free_device(void *ptr);
+/* page-locked host memory (staging for asynchronous copies) */
+int sfft_malloc_host(void **ptr, size_t bytes);
+int sfft_free_host(void *ptr);
+int sfft_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int sfft_stream_synchronize(void *stream);
+
 /* ---- synthetic inputs (seeded, counter-based splitmix64) ---- */
 
 /* dist: 0 = U[-1, 1), 1 = N(0, 1) (Box-Muller); count values of the
