@@ -31,6 +31,7 @@ EXPORTED = [
     "sfft_size_minimum", "sfft_size_maximum", "sfft_real_size_minimum",
     "sfft_real_size_maximum", "sfft_size_supported",
     "sfft_plan_create", "sfft_plan_destroy", "sfft_plan_get_info",
+    "sfft_plan_create_hinted", "sfft_plan_variants",
     "sfft_execute", "sfft_execute_strided",
     "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_execute",
     "sfft_batch_execute_per_size",
@@ -58,6 +59,16 @@ class PlanInfo(ctypes.Structure):
         ("transforms_per_block", ctypes.c_int),
         ("workspace_bytes", ctypes.c_size_t),
         ("table_bytes", ctypes.c_size_t),
+    ]
+
+
+class PlanHints(ctypes.Structure):
+    _fields_ = [
+        ("variant", ctypes.c_int),
+        ("nradices", ctypes.c_int),
+        ("radices", ctypes.c_int * 16),
+        ("transforms_per_block", ctypes.c_int),
+        ("split", ctypes.c_size_t),
     ]
 
 
@@ -95,6 +106,9 @@ def lib() -> ctypes.CDLL:
         L.sfft_size_supported.argtypes = [sz, ip]
         L.sfft_plan_create.argtypes = [ctypes.POINTER(vp), sz, ip, ip]
         L.sfft_plan_destroy.argtypes = [vp]
+        L.sfft_plan_create_hinted.argtypes = [ctypes.POINTER(vp), sz, ip, ip,
+                                              ctypes.POINTER(PlanHints)]
+        L.sfft_plan_variants.argtypes = [sz, ip, ip]
         L.sfft_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
         L.sfft_execute.argtypes = [vp, vp, vp, sz, ip, vp]
         L.sfft_execute_strided.argtypes = [vp, vp, sz, vp, sz, sz, ip, vp]

@@ -37,9 +37,24 @@ def _stream_handle(stream=None) -> int:
 class Plan:
     """An engine plan: (n, kind, precision) -> device twiddles + launch layout."""
 
-    def __init__(self, n: int, kind: int = L.C2C, precision: int = L.F32):
+    def __init__(self, n: int, kind: int = L.C2C, precision: int = L.F32, variant=None,
+                 radices=None, transforms_per_block: int = 0, split: int = 0):
+        """variant / radices / transforms_per_block / split override the
+        planner (sfft_plan_create_hinted); without them the cached plan is used."""
         h = ctypes.c_void_p()
-        L.check(L.lib().sfft_plan_create(ctypes.byref(h), n, kind, precision))
+        if variant is None and radices is None and not transforms_per_block and not split:
+            L.check(L.lib().sfft_plan_create(ctypes.byref(h), n, kind, precision))
+        else:
+            hints = L.PlanHints()
+            hints.variant = -1 if variant is None else int(variant)
+            rs = list(radices or [])
+            hints.nradices = len(rs)
+            for i, r in enumerate(rs):
+                hints.radices[i] = int(r)
+            hints.transforms_per_block = int(transforms_per_block)
+            hints.split = int(split)
+            L.check(L.lib().sfft_plan_create_hinted(ctypes.byref(h), n, kind, precision,
+                                                    ctypes.byref(hints)))
         self._h = h
         self.n, self.kind, self.precision = n, kind, precision
 

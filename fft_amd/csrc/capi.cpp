@@ -87,6 +87,29 @@ int sfft_plan_create(sfft_plan *plan, size_t n, int kind, int precision) {
     });
 }
 
+int sfft_plan_create_hinted(sfft_plan *plan, size_t n, int kind, int precision,
+                            const sfft_plan_hints *hints) {
+    return guarded([&] {
+        if (!plan) return fail(SFFT_ERR_INVALID_ARG, "plan pointer is NULL");
+        *plan = nullptr;
+        sfft::PlanHints h;
+        if (hints) {
+            h.fixed_variant = hints->variant;
+            for (int i = 0; i < hints->nradices && i < 16; ++i) h.radices.push_back(hints->radices[i]);
+            h.tpw = hints->transforms_per_block;
+            h.split = hints->split;
+        }
+        auto p = std::make_shared<sfft::Plan>(n, kind, precision, &h);
+        *plan = new sfft_plan_s{p};
+        return SFFT_OK;
+    });
+}
+
+int sfft_plan_variants(size_t n, int kind, int precision) {
+    const size_t cn = kind == SFFT_C2C ? n : n / 2;
+    return sfft::fixed_variant_count((int)cn, precision, kind == SFFT_C2C ? 0 : 1);
+}
+
 int sfft_plan_destroy(sfft_plan plan) {
     delete plan;
     return SFFT_OK;

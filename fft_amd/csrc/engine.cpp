@@ -264,9 +264,12 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     const bool force_one = h && h->split == 1;
     bool ok = false;
     // A compile-time planned kernel fixes the radices of this length.
-    const FixedKernel *fk = (h && !h->radices.empty())
+    const FixedKernel *fk = (h && (!h->radices.empty() || h->fixed_variant == -2))
                                 ? nullptr
-                                : find_fixed((int)cn_, prec, kind != kC2C ? 1 : 0);
+                                : find_fixed((int)cn_, prec, kind != kC2C ? 1 : 0,
+                                             h ? h->fixed_variant : -1);
+    if (h && h->fixed_variant >= 0 && !fk)
+        throw Error(SFFT_ERR_INVALID_ARG, "no such compiled variant");
     if (fk && !force_two) {
         PlanHints fh;
         if (h) fh = *h;
@@ -360,7 +363,7 @@ void Plan::build_tables() {
             StageDesc &st = p.pd.st[s];
             if (st.ns > 1) {
                 st.tw_off = (int32_t)total;
-                total += (size_t)st.ns * (st.radix - 1);
+                total += (size_t)st.ns * tw_group(st.radix);
             }
         }
     }

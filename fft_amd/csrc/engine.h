@@ -63,6 +63,7 @@ struct PlanHints {
     int tpw = 0;               // slots per block (0: planner's)
     int ept = 0;               // elements-per-thread instance (0: planner's)
     size_t split = 0;          // N1 of a two-pass plan (0: planner's; 1: force single pass)
+    int fixed_variant = -1;    // -1 default, -2 no fixed kernel, k >= 0 that variant
 };
 
 class Plan {

@@ -43,9 +43,27 @@ __device__ __forceinline__ void flip_im(C2<double> &x, uint32_t m) {
     x.y = __longlong_as_double(u);
 }
 
-template <typename T_, int N, int TPT, int TPW, int MODE, int... Rs>
+// Schedule knobs of a compiled plan:
+//   PFS   stage that issues the next group's loads (>= 0: that stage,
+//         -1 the last, -2 the one before)
+//   MINW  minimum waves per SIMD the register allocation must allow (0: the
+//         compiler's choice) -- more resident blocks per CU, fewer registers
+//   PAIR  load twiddles two per 16-byte load (else one per load)
+//   PF_FIRST  in the prefetch stage, issue the prefetch before (1) or after
+//         (0) the next stage's twiddle loads
+template <int PFS_, int MINW_, int PAIR_ = 1, int PF_FIRST_ = 0>
+struct Sched {
+    static constexpr int PFS = PFS_;
+    static constexpr int MINW = MINW_;
+    static constexpr int PAIR = PAIR_;
+    static constexpr int PF_FIRST = PF_FIRST_;
+};
+
+template <typename T_, int N, int TPT, int TPW, int MODE, class SCH, int... Rs>
 struct Fixed {
     using T = T_;
+    static constexpr int PFS = SCH::PFS;
+    static constexpr int MINW = SCH::MINW;
     static constexpr int NST = sizeof...(Rs);
     static constexpr int RAD[NST] = {Rs...};
     static constexpr int THREADS = TPT * TPW;
@@ -62,12 +80,60 @@ struct Fixed {
     static constexpr int tw_of(int s) {
         int o = 0;
         for (int i = 0; i < s; ++i)
-            if (ns_of(i) > 1) o += ns_of(i) * (RAD[i] - 1);
+            if (ns_of(i) > 1) o += ns_of(i) * tw_group(RAD[i]);
         return o;
     }
     static constexpr int bfly_of(int s) { return (N / RAD[s] + TPT - 1) / TPT; }
     static constexpr int PF = bfly_of(0) * RAD[0];
+    // The next group's first-stage loads are issued in stage PF_STAGE, after
+    // the twiddle loads of that stage and of the next one: vector-memory
+    // counters drain in issue order, so waiting for a twiddle never waits for
+    // the (slower) HBM prefetch.
+    static constexpr int PF_STAGE =
+        PFS >= 0 ? (PFS < NST ? PFS : NST - 1) : ((NST + PFS) > 0 ? (NST + PFS) : 0);
     static_assert(ns_of(NST) == N, "radices must multiply to N");
+
+    // twiddles of stage S for this thread's butterflies
+    template <int S, bool = (S < NST)>
+    struct TwBuf {
+        C2<T> w[1][2];
+    };
+    template <int S>
+    struct TwBuf<S, true> {
+        C2<T> w[bfly_of(S)][RAD[S] > 1 ? RAD[S] : 2];
+    };
+
+    template <int S>
+    static __device__ __forceinline__ void load_tw(const FixedArgs<T> &a, TwBuf<S> &tb,
+                                                   bool active, int lid) {
+        if constexpr (S < NST) {
+            constexpr int R = RAD[S], NS = ns_of(S), NB = N / R, B = bfly_of(S);
+            constexpr int TWG = tw_group(R);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = lid + b * TPT;
+                    if (active && ((NB % TPT) == 0 || j < NB)) {
+                        const int k = j % NS;
+                        const C2<T> *wp = a.tw + (tw_of(S) + k * TWG);
+                        if constexpr (sizeof(T) == 4 && SCH::PAIR && (TWG % 2) == 0) {
+                            // 16-byte aligned groups: two twiddles per load
+                            const float4 *w4 = reinterpret_cast<const float4 *>(wp);
+#pragma unroll
+                            for (int p = 0; p < TWG / 2; ++p) {
+                                const float4 q = w4[p];
+                                if (2 * p + 1 < R) tb.w[b][2 * p + 1] = {q.x, q.y};
+                                if (2 * p + 2 < R) tb.w[b][2 * p + 2] = {q.z, q.w};
+                            }
+                        } else {
+#pragma unroll
+                            for (int r = 1; r < R; ++r) tb.w[b][r] = wp[r - 1];
+                        }
+                    }
+                }
+            }
+        }
+    }
 
     // stage-0 element loads of slot g into pf (ROW mapping, contiguous batch)
     static __device__ __forceinline__ void load0(const FixedArgs<T> &a, C2<T> (&pf)[PF], int g,
@@ -87,36 +153,24 @@ struct Fixed {
         }
     }
 
-    template <int S>
+    // Stage S.  TW_READY: its twiddles were loaded by the previous stage.
+    // PF_HERE: also load stage S+1's twiddles into tn, then issue the next
+    // group's first-stage loads into pf.
+    template <int S, bool TW_READY, bool PF_HERE>
     static __device__ __forceinline__ void stage(const FixedArgs<T> &a, char *lds_c, int g,
                                                  bool active, int slot, int lid, C2<T> (&pf)[PF],
-                                                 int g_next) {
+                                                 int g_next, TwBuf<S> &tb, TwBuf<S + 1> &tn) {
         constexpr int R = RAD[S], NS = ns_of(S), NB = N / R, B = bfly_of(S);
         constexpr bool FIRST = (S == 0), LAST = (S == NST - 1);
         constexpr bool SRC_PF = FIRST && MODE != kModeC2R;
         constexpr bool DST_G = LAST && MODE != kModeR2C;
         constexpr bool SEP_R = ct_pow2(NB) || (NB % 16) == 0;
         constexpr bool SEP_W = (NS % 16) == 0 || (ct_pow2(NS) && ct_pow2(R));
-        constexpr int TWO = tw_of(S);
         const uint32_t lsb = (uint32_t)(slot * STRIDE) * EB;
 
         auto jvalid = [&](int b) { return (NB % TPT) == 0 || lid + b * TPT < NB; };
 
         C2<T> v[B][R];
-        // twiddles (issued before the LDS phase; 16 or fewer per thread)
-        C2<T> w[B][R];
-        if constexpr (NS > 1) {
-#pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const int j = lid + b * TPT;
-                if (active && jvalid(b)) {
-                    const int k = j % NS;
-                    const C2<T> *wp = a.tw + (TWO + k * (R - 1));
-#pragma unroll
-                    for (int r = 1; r < R; ++r) w[b][r] = wp[r - 1];
-                }
-            }
-        }
         if constexpr (SRC_PF) {
 #pragma unroll
             for (int b = 0; b < B; ++b)
@@ -125,6 +179,18 @@ struct Fixed {
                     v[b][r] = pf[b * R + r];
                     flip_im(v[b][r], a.flip_in);
                 }
+        }
+        if constexpr (!TW_READY) load_tw<S>(a, tb, active, lid);
+        if constexpr (PF_HERE) {
+            if constexpr (SCH::PF_FIRST) {
+                if (MODE != kModeC2R && g_next >= 0) load0(a, pf, g_next, lid);
+                load_tw<S + 1>(a, tn, active, lid);
+            } else {
+                load_tw<S + 1>(a, tn, active, lid);
+                if (MODE != kModeC2R && g_next >= 0) load0(a, pf, g_next, lid);
+            }
+        }
+        if constexpr (SRC_PF) {
             if (MODE == kModeR2C && a.half_shift) {
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
@@ -135,7 +201,6 @@ struct Fixed {
                     }
                 }
             }
-            if (g_next >= 0) load0(a, pf, g_next, lid);
         } else {
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -162,7 +227,7 @@ struct Fixed {
             if (active && jvalid(b)) {
                 if constexpr (NS > 1) {
 #pragma unroll
-                    for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[b][r]);
+                    for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tb.w[b][r]);
                 }
                 dft<R, T>(v[b]);
             }
@@ -209,13 +274,16 @@ struct Fixed {
         }
     }
 
-    template <int S>
+    template <int S, bool TW_READY>
     static __device__ __forceinline__ void stages_from(const FixedArgs<T> &a, char *lds_c, int g,
                                                        bool active, int slot, int lid,
-                                                       C2<T> (&pf)[PF], int g_next) {
+                                                       C2<T> (&pf)[PF], int g_next,
+                                                       TwBuf<S> &tb) {
         if constexpr (S < NST) {
-            stage<S>(a, lds_c, g, active, slot, lid, pf, g_next);
-            stages_from<S + 1>(a, lds_c, g, active, slot, lid, pf, g_next);
+            TwBuf<S + 1> tn;
+            constexpr bool PF_HERE = (S == PF_STAGE);
+            stage<S, TW_READY, PF_HERE>(a, lds_c, g, active, slot, lid, pf, g_next, tb, tn);
+            stages_from<S + 1, PF_HERE>(a, lds_c, g, active, slot, lid, pf, g_next, tn);
         }
     }
 
@@ -287,14 +355,16 @@ struct Fixed {
             const int gn = grp + (int)gridDim.x;
             const int g_next = gn < ngroups ? gn * TPW + slot : -1;
             if constexpr (MODE == kModeC2R) real_pre(a, lds_c, g, active, slot, lid);
-            stages_from<0>(a, lds_c, g, active, slot, lid, pf, g_next);
+            TwBuf<0> t0;
+            stages_from<0, false>(a, lds_c, g, active, slot, lid, pf, g_next, t0);
             if constexpr (MODE == kModeR2C) real_post(a, lds_c, g, active, slot, lid);
         }
     }
 };
 
 template <class F>
-__global__ void __launch_bounds__(F::THREADS) k_fixed(FixedArgs<typename F::T> a) {
+__global__ void __launch_bounds__(F::THREADS, F::MINW > 0 ? F::MINW : 1)
+    k_fixed(FixedArgs<typename F::T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     F::run(a, reinterpret_cast<char *>(smem_raw));
 }
@@ -302,6 +372,7 @@ __global__ void __launch_bounds__(F::THREADS) k_fixed(FixedArgs<typename F::T> a
 // Registry entry of a compiled fixed kernel.
 struct FixedKernel {
     int n;          // complex length of the core transform
+    int variant;    // schedule variant (0 = default)
     int prec;       // 0 F32, 1 F64
     int real;       // 0: C2C kernel; 1: real pair (R2C forward + C2R inverse)
     int tpt, tpw;
@@ -314,7 +385,9 @@ struct FixedKernel {
 };
 
 void register_fixed(const FixedKernel &k);
-const FixedKernel *find_fixed(int n, int prec, int real);
+// variant < 0: the default (variant 0); returns nullptr if none
+const FixedKernel *find_fixed(int n, int prec, int real, int variant = -1);
+int fixed_variant_count(int n, int prec, int real);
 
 template <class F>
 hipError_t fixed_launch_one(const void *args, unsigned grid, hipStream_t stream) {
@@ -354,12 +427,13 @@ inline void fixed_fill_radices(FixedKernel &k) {
 
 // Registers the complex kernel of one compile-time plan (namespace scope
 // object in a .hip file).
-template <typename T, int N, int TPT, int TPW, int... Rs>
+template <typename T, int N, int TPT, int TPW, class SCH, int VARIANT, int... Rs>
 struct FixedC2C {
-    using F = Fixed<T, N, TPT, TPW, kModeC2C, Rs...>;
+    using F = Fixed<T, N, TPT, TPW, kModeC2C, SCH, Rs...>;
     FixedC2C() {
         FixedKernel k{};
         k.n = N;
+        k.variant = VARIANT;
         k.prec = sizeof(T) == 8 ? 1 : 0;
         k.real = 0;
         k.tpt = TPT;
@@ -376,13 +450,14 @@ struct FixedC2C {
 
 // Registers the real forward / inverse pair of one compile-time plan of the
 // half-length complex core N (real length 2N).
-template <typename T, int N, int TPT, int TPW, int... Rs>
+template <typename T, int N, int TPT, int TPW, class SCH, int VARIANT, int... Rs>
 struct FixedReal {
-    using FR = Fixed<T, N, TPT, TPW, kModeR2C, Rs...>;
-    using FI = Fixed<T, N, TPT, TPW, kModeC2R, Rs...>;
+    using FR = Fixed<T, N, TPT, TPW, kModeR2C, SCH, Rs...>;
+    using FI = Fixed<T, N, TPT, TPW, kModeC2R, SCH, Rs...>;
     FixedReal() {
         FixedKernel k{};
         k.n = N;
+        k.variant = VARIANT;
         k.prec = sizeof(T) == 8 ? 1 : 0;
         k.real = 1;
         k.tpt = TPT;

@@ -20,11 +20,20 @@ void register_fixed(const FixedKernel &k) {
     registry().push_back(k);
 }
 
-const FixedKernel *find_fixed(int n, int prec, int real) {
+const FixedKernel *find_fixed(int n, int prec, int real, int variant) {
     std::lock_guard<std::mutex> lock(registry_mutex());
+    const int want = variant < 0 ? 0 : variant;
     for (const auto &k : registry())
-        if (k.n == n && k.prec == prec && k.real == real) return &k;
+        if (k.n == n && k.prec == prec && k.real == real && k.variant == want) return &k;
     return nullptr;
+}
+
+int fixed_variant_count(int n, int prec, int real) {
+    std::lock_guard<std::mutex> lock(registry_mutex());
+    int c = 0;
+    for (const auto &k : registry())
+        if (k.n == n && k.prec == prec && k.real == real) ++c;
+    return c;
 }
 
 }  // namespace sfft

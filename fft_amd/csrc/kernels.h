@@ -15,7 +15,7 @@ template <typename T>
 hipError_t launch_pass(const PassArgs<T> &a, int ept, unsigned grid, unsigned block,
                        size_t lds_bytes, hipStream_t stream);
 
-// Stage twiddles W_{ns R}^{r k} for k < ns, r = 1..R-1 at dst[off + k (R-1) + r - 1],
+// Stage twiddles W_{ns R}^{r k} for k < ns, r = 1..R-1 at dst[off + k tw_group(R) + r - 1],
 // computed in FP64 (sincospi) and rounded once to T.
 template <typename T>
 hipError_t gen_stage_twiddles(C2<T> *dst, int off, int ns, int radix, hipStream_t stream);

@@ -69,7 +69,7 @@ __global__ void k_stage_tw(C2<T> *dst, int off, int ns, int radix) {
     const int k = idx / (radix - 1);
     const int r = idx % (radix - 1) + 1;
     const C2<double> w = unit_root((uint64_t)r * (uint64_t)k, (uint64_t)ns * (uint64_t)radix);
-    dst[off + idx] = {(T)w.x, (T)w.y};
+    dst[off + k * tw_group(radix) + (r - 1)] = {(T)w.x, (T)w.y};
 }
 
 template <typename T>

@@ -36,10 +36,16 @@ inline FastDiv make_fastdiv(uint32_t d) {
     return f;
 }
 
+// Twiddles of one butterfly (r = 1..R-1) are stored densely, R-1 entries per
+// group.  (Groups padded to an even length for 16-byte pair loads were
+// measured slower: the larger table costs more in cache than the wider
+// loads save.)
+constexpr int tw_group(int radix) { return radix - 1; }
+
 struct StageDesc {
     int32_t radix;
     int32_t ns;      // span before this stage
-    int32_t tw_off;  // offset of this stage's twiddles (ns * (radix - 1) entries)
+    int32_t tw_off;  // offset of this stage's twiddles (ns groups of tw_group(radix))
     int32_t pad;
     FastDiv div;     // division by ns
 };

@@ -151,7 +151,7 @@ struct Block {
                 if (ns > 1) {
                     const uint32_t q = fdiv((uint32_t)j, st.div);
                     const int k = j - (int)q * ns;
-                    const C2<T> *w = a.tw + st.tw_off + k * (R - 1);
+                    const C2<T> *w = a.tw + st.tw_off + k * tw_group(R);
 #pragma unroll
                     for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r - 1]);
                 }

@@ -103,6 +103,23 @@ int sfft_plan_create(sfft_plan *plan, size_t n, int kind, int precision);
 int sfft_plan_destroy(sfft_plan plan);
 int sfft_plan_get_info(sfft_plan plan, sfft_plan_info *info);
 
+/* Planner overrides (used by the autotuner; no counterpart in the reference,
+ * whose plan is fixed by signalsmith-fft.h:93-185). */
+typedef struct {
+    int variant;          /* compile-time planned kernel schedule: -1 default,
+                             -2 none (runtime-planned kernel), k >= 0 that variant */
+    int nradices;         /* > 0: radix order of the runtime-planned pass */
+    int radices[16];
+    int transforms_per_block; /* 0: planner's choice */
+    size_t split;         /* two-pass N1 (0: planner's choice, 1: force one pass) */
+} sfft_plan_hints;
+
+/* A plan built with hints (not cached, not used by sfft_plan_create). */
+int sfft_plan_create_hinted(sfft_plan *plan, size_t n, int kind, int precision,
+                            const sfft_plan_hints *hints);
+/* Number of compiled fixed-plan schedules for (n, kind, precision). */
+int sfft_plan_variants(size_t n, int kind, int precision);
+
 /* FFT<V>::fft / ifft and RealFFT<V,flags>::fft / ifft
  * (signalsmith-fft.h:374-386, :446-502) over a batch of device buffers.
  *   C2C:       in, out: batch x n complex
