@@ -35,6 +35,9 @@ EXPORTED = [
     "sfft_execute", "sfft_execute_strided",
     "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_execute",
     "sfft_batch_execute_per_size",
+    "sfft_server_create", "sfft_server_destroy", "sfft_server_submit", "sfft_server_poll",
+    "sfft_server_drain", "sfft_server_fetch", "sfft_server_release", "sfft_server_now",
+    "sfft_server_run_poisson",
     "sfft_malloc_device", "sfft_free_device", "sfft_malloc_host", "sfft_free_host",
     "sfft_memcpy", "sfft_stream_synchronize",
     "sfft_fill",
@@ -83,6 +86,50 @@ class BatchItem(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_size_t),
+        ("max_inflight", ctypes.c_int),
+        ("streams", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("plain", ctypes.c_int),
+    ]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("n", ctypes.c_size_t),
+        ("kind", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("batch", ctypes.c_size_t),
+        ("seed", ctypes.c_uint64),
+        ("retain", ctypes.c_int),
+    ]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("status", ctypes.c_int),
+        ("t_arrive", ctypes.c_double),
+        ("t_admit", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+        ("bytes", ctypes.c_size_t),
+    ]
+
+
+class PoissonConfig(ctypes.Structure):
+    _fields_ = [
+        ("rate", ctypes.c_double),
+        ("duration", ctypes.c_double),
+        ("n_min", ctypes.c_size_t),
+        ("n_max", ctypes.c_size_t),
+        ("batch_max", ctypes.c_size_t),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
 _lib = None
 
 
@@ -117,6 +164,17 @@ def lib() -> ctypes.CDLL:
         L.sfft_batch_execute.argtypes = [vp, vp, vp, vp]
         L.sfft_batch_execute_per_size.argtypes = [vp, vp, vp, vp]
         L.sfft_fill.argtypes = [vp, sz, ip, ip, ctypes.c_uint64, vp]
+        L.sfft_server_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ServerConfig)]
+        L.sfft_server_destroy.argtypes = [vp]
+        L.sfft_server_submit.argtypes = [vp, ctypes.POINTER(Request)]
+        L.sfft_server_poll.argtypes = [vp, ctypes.POINTER(Completion), sz, ctypes.POINTER(sz)]
+        L.sfft_server_drain.argtypes = [vp]
+        L.sfft_server_fetch.argtypes = [vp, ctypes.c_uint64, vp, sz]
+        L.sfft_server_release.argtypes = [vp, ctypes.c_uint64]
+        L.sfft_server_now.argtypes = [vp]
+        L.sfft_server_now.restype = ctypes.c_double
+        L.sfft_server_run_poisson.argtypes = [vp, ctypes.POINTER(PoissonConfig),
+                                              ctypes.POINTER(Completion), sz, ctypes.POINTER(sz)]
         _lib = L
     return _lib
 

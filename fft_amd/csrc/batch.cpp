@@ -81,7 +81,8 @@ void UnevenBatch::build(size_t lds_cap) {
                 group_cost[e0] = (double)(in_step + out_step);
             } else {
                 HostUnit h;
-                h.u = WorkUnit{e0, npass, 1, -1, (int64_t)io, (int64_t)oo};
+                const int post = (it.kind != SFFT_C2C && inv == 0) ? kUnitRealPost : 0;
+                h.u = WorkUnit{e0, npass | post, 1, -1, (int64_t)io, (int64_t)oo};
                 h.cost = (double)(in_step + out_step) * npass;
                 hu.push_back(h);
             }

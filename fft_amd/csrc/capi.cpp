@@ -8,19 +8,24 @@
 #include "batch.h"
 #include "engine.h"
 
+namespace sfft {
+std::string &last_error() {
+    thread_local std::string msg;
+    return msg;
+}
+}  // namespace sfft
+
 namespace {
 
-thread_local std::string g_last_error;
-
 int fail(int code, const std::string &msg) {
-    g_last_error = msg;
+    sfft::last_error() = msg;
     return code;
 }
 
 template <typename F>
 int guarded(F &&f) {
     try {
-        g_last_error.clear();
+        sfft::last_error().clear();
         return f();
     } catch (const sfft::Error &e) {
         return fail(e.code, e.what());
@@ -50,7 +55,7 @@ int sfft_init(void) {
     });
 }
 
-const char *sfft_last_error(void) { return g_last_error.c_str(); }
+const char *sfft_last_error(void) { return sfft::last_error().c_str(); }
 
 int sfft_device_info(int *cu_count, size_t *lds_bytes, char *arch, size_t arch_len) {
     return guarded([&] {

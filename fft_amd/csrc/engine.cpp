@@ -292,9 +292,6 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         passes_.push_back(single);
     } else {
         if (force_one) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "does not fit one pass");
-        if (kind != kC2C)
-            throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
-                        "real transforms above the single-pass limit are not planned yet");
         // Two passes: cn = N1 * N2.  Pass 1: N2 column FFTs of length N1
         // (stride N2) with W_N^{n2 k1}, written transposed as rows; pass 2:
         // N1 column FFTs of length N2 in place (stride N1).
@@ -443,7 +440,18 @@ PassArgs<T> Plan::pass_args(size_t p, int inverse) const {
     a.tw = (const C2<T> *)d_tw_;
     a.conj_in = inverse && p == 0;
     a.conj_out = inverse && p + 1 == np;
-    a.mode = kind_ == kC2C ? kModeC2C : (inverse ? kModeC2R : kModeR2C);
+    if (kind_ == kC2C) {
+        a.mode = kModeC2C;
+    } else if (np == 1) {
+        a.mode = inverse ? kModeC2R : kModeR2C;
+    } else if (p == 0) {
+        // two-pass real: pre-process / rotation fused into the first pass' loads
+        a.mode = inverse ? kModeC2RLoad : kModeR2CLoad;
+        if (inverse) a.conj_in = 0;  // the fused pre-process stores conj(.) itself
+    } else {
+        a.mode = inverse ? kModeC2RStore : kModeC2C;
+    }
+    a.real_h = (int32_t)cn_;
     a.half_shift = kind_ == kR2CHalf;
     a.post_tw = pp.post_tw;
     a.post_tw_shift = ptw_shift_;

@@ -131,4 +131,7 @@ bool supported_length(size_t n);
 
 void check_hip(hipError_t e, const char *what);
 
+// Message of the last failed C ABI call on this thread.
+std::string &last_error();
+
 }  // namespace sfft

@@ -301,6 +301,7 @@ struct Fixed {
                 ls[0] = {v.x + v.y, -(v.x - v.y)};
             }
             for (int i = (mod ? 0 : 1) + lid; i <= H / 2; i += TPT) {
+                if (real_pair_skipped(i, H, mod)) continue;
                 const int ci = mod ? (H - 1 - i) : (H - i);
                 const C2<T> v = x[i], v2 = x[ci];
                 const C2<T> odd = {v.x + v2.x, v.y - v2.y};
@@ -330,6 +331,7 @@ struct Fixed {
             }
             const T half = (T)0.5;
             for (int i = (mod ? 0 : 1) + lid; i <= H / 2; i += TPT) {
+                if (real_pair_skipped(i, H, mod)) continue;
                 const int ci = mod ? (H - 1 - i) : (H - i);
                 const C2<T> zi = ls[ct_pad(i)], zc = ls[ct_pad(ci)];
                 const C2<T> odd = {(zi.x + zc.x) * half, (zi.y - zc.y) * half};

@@ -15,6 +15,12 @@ template <typename T>
 hipError_t launch_pass(const PassArgs<T> &a, int ept, unsigned grid, unsigned block,
                        size_t lds_bytes, hipStream_t stream);
 
+// In-place forward real post-process of batch spectra of length h (distance
+// dist complex values) after a two-pass complex FFT.
+template <typename T>
+hipError_t launch_real_post(C2<T> *out, int64_t dist, int h, size_t batch, const C2<T> *tmi,
+                            int half_shift, hipStream_t stream);
+
 // Stage twiddles W_{ns R}^{r k} for k < ns, r = 1..R-1 at dst[off + k tw_group(R) + r - 1],
 // computed in FP64 (sincospi) and rounded once to T.
 template <typename T>

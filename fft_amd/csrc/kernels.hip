@@ -51,6 +51,34 @@ hipError_t launch_pass<double>(const PassArgs<double> &a, int ept, unsigned grid
 }
 
 
+// ------------------------------------------- real post-process (two-pass) --
+
+template <typename T>
+__global__ void k_real_post(C2<T> *out, int64_t dist, int h, int batch, const C2<T> *tmi,
+                            int mod) {
+    const int t0 = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int nt = (int)(gridDim.x * blockDim.x);
+    for (int b = (int)blockIdx.y; b < batch; b += (int)gridDim.y)
+        real_post_inplace(out + (int64_t)b * dist, tmi, h, mod != 0, t0, nt);
+}
+
+template <typename T>
+hipError_t launch_real_post(C2<T> *out, int64_t dist, int h, size_t batch, const C2<T> *tmi,
+                            int half_shift, hipStream_t stream) {
+    if (batch == 0) return hipSuccess;
+    const int pairs = h / 2 + 1;
+    unsigned gx = (unsigned)((pairs + 255) / 256);
+    if (gx > 64) gx = 64;
+    const unsigned gy = (unsigned)(batch < 65535 ? batch : 65535);
+    hipLaunchKernelGGL((k_real_post<T>), dim3(gx, gy), dim3(256), 0, stream, out, dist, h,
+                       (int)batch, tmi, half_shift);
+    return hipGetLastError();
+}
+template hipError_t launch_real_post<float>(C2<float> *, int64_t, int, size_t, const C2<float> *,
+                                            int, hipStream_t);
+template hipError_t launch_real_post<double>(C2<double> *, int64_t, int, size_t,
+                                             const C2<double> *, int, hipStream_t);
+
 // --------------------------------------------------------------- twiddles --
 
 // e^{-2 pi i m / L} for an exact integer ratio, in FP64.

@@ -21,6 +21,8 @@ struct WorkUnit {
 };
 
 constexpr int kPersistentBlock = 1024;
+// WorkUnit.npasses bit: forward real two-pass unit, post-process after pass 2
+constexpr int32_t kUnitRealPost = 0x100;
 
 // Launches the persistent kernel: `grid` workgroups pull units from a
 // device counter until the list is exhausted; the last workgroup to finish

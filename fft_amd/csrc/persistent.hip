@@ -23,7 +23,8 @@ __global__ void __launch_bounds__(kPersistentBlock)
         const bool gathered = wu.gather >= 0;
         const C2<T> *in = reinterpret_cast<const C2<T> *>(in_arena + (gathered ? 0 : wu.in_off));
         C2<T> *out = reinterpret_cast<C2<T> *>(out_arena + (gathered ? 0 : wu.out_off));
-        for (int p = 0; p < wu.npasses; ++p) {
+        const int npasses = wu.npasses & 0xff;
+        for (int p = 0; p < npasses; ++p) {
             const PassArgs<T> &a = table[wu.args + p];
             const int spt = (int)a.iin.per_batch.d > 1 ? (int)a.iin.per_batch.d : 1;
             const int total = wu.count * spt;
@@ -36,10 +37,16 @@ __global__ void __launch_bounds__(kPersistentBlock)
                 }
                 blk.run();
             }
-            if (p + 1 < wu.npasses) {
-                __threadfence();  // pass 1 stores visible before pass 2 loads
+            if (p + 1 < npasses || (wu.npasses & kUnitRealPost)) {
+                __threadfence();  // this pass' stores visible before the next loads
                 __syncthreads();
             }
+        }
+        if (wu.npasses & kUnitRealPost) {
+            const PassArgs<T> &a = table[wu.args + npasses - 1];
+            real_post_inplace(out, a.real_tw, a.real_h, a.half_shift != 0, tid,
+                              (int)blockDim.x);
+            __syncthreads();
         }
     }
     // Every workgroup has made its final (failing) grab; the last one to

@@ -24,7 +24,20 @@ enum : int32_t {
     kModeC2C = 0,
     kModeR2C = 1,  // forward real: post-process pairs (i, h-i) after the last stage
     kModeC2R = 2,  // inverse real: pre-process pairs before the first stage
+    // two-pass real transforms (global element index m within the transform):
+    kModeR2CLoad = 3,   // first pass: half-shift rotation at the HBM load
+    kModeC2RLoad = 4,   // first pass: pre-process from in[i], in[h-i] at the HBM load
+    kModeC2RStore = 5,  // second pass: half-shift rotation at the HBM store
 };
+
+// The pair loops of the real pre-/post-process (signalsmith-fft.h:463-472,
+// :482-492) visit i = 0..h/2 (1..h/2 without the half shift) with partner
+// ci = h-1-i (h-i).  With the half shift and h even the pairs i = h/2-1 and
+// h/2 are the same two bins, and the reference keeps the values of the later
+// one: that pair is skipped so every bin has exactly one writer.
+__device__ __forceinline__ bool real_pair_skipped(int i, int h, bool mod) {
+    return mod && (h % 2 == 0) && i == h / 2 - 1;
+}
 
 template <typename T>
 struct PassArgs {
@@ -45,8 +58,34 @@ struct PassArgs {
     const C2<double> *ptw_fine;    // W_N^{m}, m < 2^shift
     const C2<double> *ptw_coarse;  // W_N^{m 2^shift}
     int32_t in_from_out;   // second pass of a two-pass plan: reads the output buffer
-    int32_t pad_;
+    int32_t real_h;        // complex length h of a two-pass real transform
 };
+
+// Value of pre-processed bin m (before the inverse complex FFT) from the
+// half spectrum x of length h (signalsmith-fft.h:478-492), for one element.
+template <typename T>
+__device__ __forceinline__ C2<T> real_pre_bin(const C2<T> *x, const C2<T> *tmi, int m, int h,
+                                             bool mod) {
+    if (!mod && m == 0) {
+        const C2<T> v = x[0];
+        return {v.x + v.y, v.x - v.y};
+    }
+    int i, ci;
+    if (mod && (h % 2 == 0) && (m == h / 2 - 1 || m == h / 2)) {
+        i = h / 2;
+        ci = h / 2 - 1;
+    } else {
+        const int mirror = mod ? (h - 1 - m) : (h - m);
+        i = m < mirror ? m : mirror;
+        ci = mod ? (h - 1 - i) : (h - i);
+    }
+    const C2<T> v = x[i], v2 = x[ci];
+    const C2<T> odd = {v.x + v2.x, v.y - v2.y};
+    const C2<T> erot = {v.x - v2.x, v.y + v2.y};
+    const C2<T> ev = cmulc(erot, tmi[i]);
+    if (m == i && i != ci) return cadd(odd, ev);
+    return cconj(csub(odd, ev));
+}
 
 // The pass descriptor `a` may live in kernel arguments or in a device table
 // (persistent launches); in, out and total_slots are per launch / per item.
@@ -73,6 +112,19 @@ struct Block {
     }
     __device__ __forceinline__ int64_t obase(int g) const {
         return gather_out ? gather_out[g] : gbase(a.iout, g);
+    }
+
+    // transform base and in-transform offset of slot g's element 0
+    __device__ __forceinline__ void split_slot(const IoDesc &io, int g, int64_t &tbase,
+                                               int64_t &cofs) const {
+        if (io.per_batch.d <= 1) {
+            tbase = (int64_t)g * io.batch_stride;
+            cofs = 0;
+            return;
+        }
+        const uint32_t b = fdiv((uint32_t)g, io.per_batch);
+        tbase = (int64_t)b * io.batch_stride;
+        cofs = (int64_t)((uint32_t)g - b * io.per_batch.d) * io.slot_stride;
     }
 
     __device__ __forceinline__ int64_t gbase(const IoDesc &io, int g) const {
@@ -114,9 +166,28 @@ struct Block {
 
         C2<T> v[B][R];
         // ---- load
-        if (src_global) {
+        if (src_global && a.mode == kModeC2RLoad) {
+            // two-pass inverse real: pre-process each element from its pair
+            int64_t tb = 0, cofs = 0;
+            if (active) split_slot(a.iin, g, tb, cofs);
+            const int64_t es = a.iin.elem_stride;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lid + b * pd.tpt;
+                if (active && j < nb) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int m = (int)(cofs + (int64_t)(j + r * nb) * es);
+                        v[b][r] = cconj(real_pre_bin(in + tb, a.real_tw, m, a.real_h,
+                                                     a.half_shift != 0));
+                    }
+                }
+            }
+        } else if (src_global) {
             const int64_t base = active ? ibase(g) : 0;
             const int64_t es = a.iin.elem_stride;
+            int64_t tb = 0, cofs = 0;
+            if (active && a.mode == kModeR2CLoad) split_slot(a.iin, g, tb, cofs);
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = lid + b * pd.tpt;
@@ -127,6 +198,8 @@ struct Block {
                         C2<T> x = in[base + (int64_t)e * es];
                         if (a.conj_in) x.y = -x.y;
                         if (a.mode == kModeR2C && a.half_shift) x = cmul(x, a.rot[e]);
+                        if (a.mode == kModeR2CLoad && a.half_shift)
+                            x = cmul(x, a.rot[cofs + (int64_t)e * es]);
                         v[b][r] = x;
                     }
                 }
@@ -162,6 +235,8 @@ struct Block {
         if (dst_global) {
             const int64_t base = active ? obase(g) : 0;
             const int64_t es = a.iout.elem_stride;
+            int64_t tbo = 0, cofo = 0;
+            if (active && a.mode == kModeC2RStore) split_slot(a.iout, g, tbo, cofo);
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = lid + b * pd.tpt;
@@ -175,6 +250,8 @@ struct Block {
                         C2<T> y = v[b][r];
                         if (a.conj_out) y.y = -y.y;
                         if (a.mode == kModeC2R && a.half_shift) y = cmulc(y, a.rot[e]);
+                        if (a.mode == kModeC2RStore && a.half_shift)
+                            y = cmulc(y, a.rot[cofo + (int64_t)e * es]);
                         if (a.post_tw) y = cmul(y, post_twiddle(g, e));
                         out[base + (int64_t)e * es] = y;
                     }
@@ -235,6 +312,7 @@ struct Block {
                 ls[0] = {v.x + v.y, -(v.x - v.y)};
             }
             for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += pd.tpt) {
+                if (real_pair_skipped(i, h, mod)) continue;
                 const int ci = mod ? (h - 1 - i) : (h - i);
                 const C2<T> v = x[i], v2 = x[ci];
                 const C2<T> odd = {v.x + v2.x, v.y - v2.y};
@@ -266,6 +344,7 @@ struct Block {
         }
         const T half = (T)0.5;
         for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += pd.tpt) {
+            if (real_pair_skipped(i, h, mod)) continue;
             const int ci = mod ? (h - 1 - i) : (h - i);
             const C2<T> zi = ls[lds_idx(i)], zc = ls[lds_idx(ci)];
             const C2<T> odd = {(zi.x + zc.x) * half, (zi.y - zc.y) * half};
@@ -295,5 +374,28 @@ struct Block {
         if (a.mode == kModeR2C) real_post_sync();
     }
 };
+
+// Forward real post-process in place over a spectrum z of length h (the
+// output of a two-pass complex FFT): pair (i, ci) is read and written by one
+// thread (signalsmith-fft.h:459-472).  Threads t0, t0 + nt, ... of the caller.
+template <typename T>
+__device__ __forceinline__ void real_post_inplace(C2<T> *z, const C2<T> *tmi, int h, bool mod,
+                                                  int t0, int nt) {
+    const T half = (T)0.5;
+    for (int i = (mod ? 0 : 1) + t0; i <= h / 2; i += nt) {
+        if (real_pair_skipped(i, h, mod)) continue;
+        const int ci = mod ? (h - 1 - i) : (h - i);
+        const C2<T> zi = z[i], zc = z[ci];
+        const C2<T> odd = {(zi.x + zc.x) * half, (zi.y - zc.y) * half};
+        const C2<T> ev = {(zi.x - zc.x) * half, (zi.y + zc.y) * half};
+        const C2<T> e = cmul(ev, tmi[i]);
+        z[i] = cadd(odd, e);
+        z[ci] = cconj(csub(odd, e));
+    }
+    if (!mod && t0 == 0) {
+        const C2<T> d = z[0];
+        z[0] = {d.x + d.y, d.x - d.y};
+    }
+}
 
 }  // namespace sfft

@@ -160,6 +160,68 @@ int sfft_batch_execute(sfft_batch batch, const void *in_arena, void *out_arena, 
 int sfft_batch_execute_per_size(sfft_batch batch, const void *in_arena, void *out_arena,
                                 void *stream);
 
+/* ---- serving: continuous batching of transform requests ----
+ * No counterpart in the reference (a single-threaded library).  Requests of
+ * any kind, size and batch are admitted in arrival order as in-flight slots
+ * and HBM budget free up; admitted requests run concurrently on several
+ * device streams.  Inputs are synthesised in HBM from the request's seed
+ * (stand-in for the client's upload); outputs stay in the server's pool. */
+
+typedef struct sfft_server_s *sfft_server;
+
+typedef struct {
+    size_t hbm_budget_bytes; /* pool for request buffers (0: 64 GiB) */
+    int max_inflight;        /* requests on the device at once (0: 64) */
+    int streams;             /* device streams (0: 4) */
+    int precision;           /* SFFT_F32 / SFFT_F64 */
+    int plain;               /* 1: serve one request at a time, synchronously */
+} sfft_server_config;
+
+typedef struct {
+    uint64_t id;
+    size_t n;
+    int kind;      /* SFFT_C2C or SFFT_R2C */
+    int direction; /* SFFT_FORWARD / SFFT_INVERSE */
+    size_t batch;
+    uint64_t seed; /* input: U[-1, 1) from this seed */
+    int retain;    /* 1: keep the output until sfft_server_release */
+} sfft_request;
+
+typedef struct {
+    uint64_t id;
+    int status;             /* SFFT_OK or an error code */
+    double t_arrive, t_admit, t_done; /* seconds, server clock */
+    size_t bytes;           /* algorithmic bytes of the request */
+} sfft_completion;
+
+int sfft_server_create(sfft_server *server, const sfft_server_config *config);
+int sfft_server_destroy(sfft_server server);
+/* Queue a request (never blocks on the device; SFFT_ERR_BUDGET if a single
+ * request exceeds the whole budget). */
+int sfft_server_submit(sfft_server server, const sfft_request *request);
+/* Completed requests since the last poll (up to max). */
+int sfft_server_poll(sfft_server server, sfft_completion *out, size_t max, size_t *count);
+/* Block until every submitted request has completed. */
+int sfft_server_drain(sfft_server server);
+/* Copy a retained output to host memory / free it. */
+int sfft_server_fetch(sfft_server server, uint64_t id, void *host_dst, size_t bytes);
+int sfft_server_release(sfft_server server, uint64_t id);
+/* Seconds on the server clock. */
+double sfft_server_now(sfft_server server);
+
+/* Built-in load generator: Poisson arrivals at `rate` req/s for `duration`
+ * seconds; each request is C2C forward, C2C inverse or R2C forward (1/3
+ * each), a fast size log-uniform in [n_min, n_max], batch uniform in
+ * [1, batch_max], inputs from per-request seeds; returns when all have
+ * completed.  records: one per request (up to max_records). */
+typedef struct {
+    double rate, duration;
+    size_t n_min, n_max, batch_max;
+    uint64_t seed;
+} sfft_poisson_config;
+int sfft_server_run_poisson(sfft_server server, const sfft_poisson_config *config,
+                            sfft_completion *records, size_t max_records, size_t *count);
+
 /* ---- memory helpers for host-side bindings (the drop-in header uses them to
  * stage host iterators; no counterpart in the reference, which is host-only) ---- */
 

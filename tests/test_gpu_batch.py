@@ -79,10 +79,10 @@ def test_graph_replays_are_stable():
 
 
 def test_mixed_kinds_in_one_launch():
-    sizes = [2048, 4096, 1920, 3000, 256, 65536, 2048, 30000, 100, 6]
+    sizes = [2048, 4096, 1920, 3000, 256, 65536, 2048, 30000, 100, 6, 65536, 80000, 48000, 96000]
     kinds = [L.R2C, L.C2C, L.R2C_HALFSHIFT, L.C2C, L.R2C, L.C2C, L.R2C, L.C2C, L.R2C_HALFSHIFT,
-             L.R2C]
-    dirs = [0, 1, 1, 0, 1, 0, 1, 1, 0, 0]
+             L.R2C, L.R2C, L.R2C, L.R2C_HALFSHIFT, L.R2C_HALFSHIFT]
+    dirs = [0, 1, 1, 0, 1, 0, 1, 1, 0, 0, 0, 1, 0, 1]
     b, xin = _make(sizes, dirs, kinds, seed=13)
     out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
     ref = torch.empty_like(out)

@@ -122,7 +122,8 @@ def test_real_config_2048_frames():
     assert rt <= 2 * TOL32, rt
 
 
-@pytest.mark.parametrize("n", [6, 10, 30, 250, 1000, 1920, 4096, 6000, 16384, 32768])
+@pytest.mark.parametrize("n", [6, 10, 30, 250, 1000, 1920, 4096, 6000, 16384, 32768,
+                               65536, 96000, 2 ** 18])
 @pytest.mark.parametrize("mod", [False, True])
 def test_real_float_sizes(n, mod):
     x = synth.real_uniform(n + 3, (3, n), np.float32)
@@ -146,3 +147,18 @@ def test_real_setters_mirror_reference():
             with pytest.raises(L.SfftError):
                 rf.setSizeMinimum(n)
         assert rf.setSizeMaximum(n) == t["real_setmax"][n - 1]
+
+
+@pytest.mark.parametrize("mod", [False, True])
+@pytest.mark.parametrize("n", [12, 100, 4096, 65536])
+def test_real_half_shift_pair_overlap_single_writer(n, mod):
+    # h = n/2 even with the half shift: bins h/2-1 and h/2 are written by one
+    # pair only; results are deterministic across repeated runs.
+    x = synth.real_uniform(n + 11, (4, n), np.float32)
+    y1 = _fwd(n, x, mod, dtype=torch.float32)
+    y2 = _fwd(n, x, mod, dtype=torch.float32)
+    assert np.array_equal(y1, y2)
+    assert po.rel_l2(y1, po.real_fft(x.astype(np.float64), half_shift=mod)) <= TOL32
+    z1 = _inv(n, y1, mod, dtype=torch.float32)
+    z2 = _inv(n, y1, mod, dtype=torch.float32)
+    assert np.array_equal(z1, z2)

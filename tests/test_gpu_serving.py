@@ -1,0 +1,92 @@
+"""Serving loop: continuous batching under an HBM budget.
+
+The same request gives the same result alone (a plain plan execution on the
+same synthetic input) and in the mix, bitwise; a budget smaller than the
+offered load applies backpressure without failures; a request larger than the
+budget is rejected with SFFT_ERR_BUDGET."""
+import numpy as np
+import pytest
+import torch
+
+import fft_amd
+from fft_amd import _lib as L
+from fft_amd.serving import Server, latency_summary
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(n, kind, direction, batch, seed):
+    """The request served alone: synthesise the same input, run the plan."""
+    plan = fft_amd.Plan(n, kind, L.F32)
+    if kind == L.C2C:
+        x = torch.empty(batch, n, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+    elif direction == L.FORWARD:
+        x = torch.empty(batch, n, dtype=torch.float32, device="cuda")
+        y = torch.empty(batch, n // 2, dtype=torch.complex64, device="cuda")
+    else:
+        x = torch.empty(batch, n // 2, dtype=torch.complex64, device="cuda")
+        y = torch.empty(batch, n, dtype=torch.float32, device="cuda")
+    fft_amd.fill(x, seed=seed)
+    plan.execute_ptr(x.data_ptr(), y.data_ptr(), batch, direction,
+                     torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), y.cpu().numpy()
+
+
+def test_same_result_alone_and_in_the_mix():
+    reqs = [(4096, L.C2C, 0, 7), (1920, L.C2C, 1, 3), (2048, L.R2C, 0, 16), (65536, L.C2C, 0, 2),
+            (3000, L.C2C, 0, 5), (1024, L.R2C, 1, 9), (16384, L.C2C, 1, 4), (6000, L.R2C, 0, 3),
+            (256, L.C2C, 0, 500), (48000, L.C2C, 1, 1)]
+    with Server(budget_bytes=1 << 30, max_inflight=6, streams=4) as srv:
+        for i, (n, k, d, b) in enumerate(reqs):
+            srv.submit(i, n, k, d, b, seed=1000 + i, retain=True)
+        srv.drain()
+        rec = srv.poll()
+        assert len(rec) == len(reqs) and (rec["status"] == 0).all()
+        for i, (n, k, d, b) in enumerate(reqs):
+            x, y = _alone(n, k, d, b, 1000 + i)
+            got = srv.fetch(i, y.nbytes).view(y.dtype).reshape(y.shape)
+            srv.release(i)
+            assert np.array_equal(got, y), (n, k, d, b)
+            # and the oracle on the first transform of the request
+            if k == L.C2C:
+                want = po.fft(x[0].astype(np.complex128), inverse=bool(d))
+            elif d == 0:
+                want = po.real_fft(x[0].astype(np.float64))
+            else:
+                want = po.real_ifft(x[0].astype(np.complex128))
+            assert po.rel_l2(got[0], want) <= 1e-6, (n, k, d)
+
+
+def test_backpressure_under_small_budget():
+    # 32 requests of 32 MiB each through a 96 MiB pool: at most a few fit at once
+    with Server(budget_bytes=96 << 20, max_inflight=16) as srv:
+        for i in range(32):
+            srv.submit(i, 16384, L.C2C, i % 2, 256, seed=i)
+        srv.drain()
+        rec = srv.poll()
+    assert len(rec) == 32 and (rec["status"] == 0).all()
+    order = np.argsort(rec["t_admit"])
+    assert list(rec["id"][order]) == list(range(32)), "admission is in arrival order"
+    s = latency_summary(rec)
+    assert s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_oversized_request_rejected():
+    with Server(budget_bytes=16 << 20) as srv:
+        srv.submit(0, 1 << 20, L.C2C, 0, 4, seed=0)  # 64 MiB of buffers
+        srv.submit(1, 1024, L.C2C, 0, 4, seed=1)
+        srv.drain()
+        rec = srv.poll()
+    st = {int(r["id"]): int(r["status"]) for r in rec}
+    assert st[0] == L.SFFT_ERR_BUDGET and st[1] == 0
+
+
+def test_poisson_run_small():
+    with Server(budget_bytes=8 << 30) as srv:
+        rec = srv.run_poisson(rate=2000, duration=0.25, n_min=256, n_max=1 << 16, batch_max=64)
+    s = latency_summary(rec)
+    assert s["requests"] > 300 and s["failed"] == 0, s
+    assert s["p50_ms"] > 0 and s["p99_ms"] >= s["p50_ms"]
