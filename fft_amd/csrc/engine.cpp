@@ -503,6 +503,11 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         const unsigned grid = (unsigned)((slots + pp.pd.tpw - 1) / pp.pd.tpw);
         check_hip(launch_pass<T>(a, pp.ept, grid, pp.block, pp.lds_bytes, stream), "pass launch");
     }
+    // two-pass forward real: pairwise post-process in place on the output
+    if (kind_ != kC2C && !inverse && np > 1)
+        check_hip(launch_real_post<T>((C2<T> *)out, dout, (int)cn_, batch,
+                                      (const C2<T> *)d_real_tw_, kind_ == kR2CHalf, stream),
+                  "real post-process launch");
 }
 
 // Compile-time planned kernel over a contiguous batch.  It addresses HBM

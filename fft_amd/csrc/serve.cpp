@@ -344,7 +344,7 @@ void run_poisson(Server &srv, const sfft_poisson_config &pc, int precision,
             for (size_t p5 = p3; p5 <= pc.n_max; p5 *= 5)
                 if (p5 >= pc.n_min) fast.push_back(p5);
     std::sort(fast.begin(), fast.end());
-    if (fast.empty()) throw Error(SFFT_ERR_INVALID_ARG, "no fast size in range");
+    if (fast.empty() || fast.back() < 2) throw Error(SFFT_ERR_INVALID_ARG, "no fast size in range");
     std::vector<double> logs;
     for (size_t f : fast) logs.push_back(std::log((double)f));
     // requests
@@ -362,9 +362,12 @@ void run_poisson(Server &srv, const sfft_poisson_config &pc, int precision,
         const double k = unit(pc.seed, 6 * i + 1);
         const double ln = std::log((double)pc.n_min) +
                           unit(pc.seed, 6 * i + 2) * (std::log((double)pc.n_max) - std::log((double)pc.n_min));
-        size_t best = 0;
-        for (size_t j = 1; j < fast.size(); ++j)
-            if (std::fabs(logs[j] - ln) < std::fabs(logs[best] - ln)) best = j;
+        const bool real = k >= 2.0 / 3;
+        size_t best = SIZE_MAX;
+        for (size_t j = 0; j < fast.size(); ++j) {
+            if (real && (fast[j] % 2)) continue;  // real transforms need even sizes
+            if (best == SIZE_MAX || std::fabs(logs[j] - ln) < std::fabs(logs[best] - ln)) best = j;
+        }
         r.n = fast[best];
         if (k < 1.0 / 3) {
             r.kind = SFFT_C2C;
