@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain-sample", type=int, default=256,
                     help="transforms per size timed on the plain per-call path")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the uneven-batch and serving measurements")
     return ap.parse_args()
 
 
@@ -152,6 +154,59 @@ def plain_per_call(torch, fft_amd, L, bufs, sample):
             total_bytes += by
             total_ms += ms
     return total_bytes / total_ms / 1e6, per
+
+
+def uneven_batch(torch, fft_amd, count=20000, reps=5):
+    """The uneven batch: 20000 transforms, fast sizes log-uniform in [64, 65536],
+    direction 50/50, U[-1,1) inputs; one persistent graph-captured launch vs the
+    plain per-size loop (one batched launch per item)."""
+    from fft_amd import workloads
+    sizes, dirs = workloads.uneven_batch(seed=3, count=count)
+    b = fft_amd.UnevenBatch([int(x) for x in sizes], [int(d) for d in dirs])
+    xin = torch.empty(b.in_bytes, dtype=torch.uint8, device="cuda")
+    fft_amd.fill(xin.view(torch.float32), seed=3)
+    out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    alg = b.in_bytes + b.out_bytes
+    res = {}
+    for name, fn in (("persistent", b.execute), ("per_size_loop", b.execute_per_size)):
+        fn(xin, out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn(xin, out)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": round(ms, 3), "GB/s": round(alg / ms / 1e6, 1)}
+    res["speedup"] = round(res["per_size_loop"]["ms"] / res["persistent"]["ms"], 2)
+    res["transforms"] = count
+    res["alg_bytes"] = alg
+    del xin, out, b
+    torch.cuda.empty_cache()
+    return res
+
+
+def serving(rates=(2000, 10000, 50000), duration=0.25):
+    """Poisson arrivals: C2C fwd / inv / RealFFT fwd (1/3 each), fast sizes
+    log-uniform in [256, 2^18], batch U{1..512}, per-request seeds; 64 GB
+    HBM budget.  p50/p99 latency (arrival -> completion) and throughput, and
+    the plain path (one request at a time, synchronous) at the lowest rate."""
+    from fft_amd.serving import Server, latency_summary
+    out = {}
+    with Server(budget_bytes=64 << 30, max_inflight=64, streams=4) as srv:
+        for rate in rates:
+            rec = srv.run_poisson(rate=rate, duration=duration, n_min=256, n_max=1 << 18,
+                                  batch_max=512, seed=5)
+            out[f"rate_{rate}"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                                   for k, v in latency_summary(rec).items()}
+    with Server(budget_bytes=64 << 30, plain=True) as srv:
+        rate = rates[0]
+        rec = srv.run_poisson(rate=rate, duration=duration, n_min=256, n_max=1 << 18,
+                              batch_max=512, seed=5)
+        out[f"plain_rate_{rate}"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                                     for k, v in latency_summary(rec).items()}
+    return out
 
 
 def cpu_baseline(torch, bufs, launches):
@@ -273,6 +328,12 @@ def main():
                      "launch_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom.alg_bytes},
         "wall_s_timed": round(wall, 3),
     }
+    if not args.no_extras:
+        result["uneven_batch"] = uneven_batch(torch, fft_amd)
+        result["serving"] = serving()
+        sv = result["serving"]
+        result["serving_p50_ms"] = sv["rate_10000"]["p50_ms"]
+        result["serving_p99_ms"] = sv["rate_10000"]["p99_ms"]
     if not args.no_cpu:
         cpu, errors = cpu_baseline(torch, bufs, launches)
         result["cpu_baseline"] = cpu

@@ -64,6 +64,17 @@ void UnevenBatch::build(size_t lds_cap) {
                     int tpw = kPersistentBlock / a.pd.tpt;
                     if (sl) tpw = std::min<int>(tpw, (int)(lds_cap / sl));
                     a.pd.tpw = std::max(1, tpw);
+                } else {
+                    // column passes: as many adjacent columns per round as the
+                    // block and LDS allow; the last round of a transform masks
+                    // the columns past its end
+                    const int spt = p.passes()[q].slots_per_transform;
+                    int tpw = kPersistentBlock / a.pd.tpt;
+                    if (sl) tpw = std::min<int>(tpw, (int)(lds_cap / sl));
+                    tpw = std::min(tpw, spt);
+                    int c = 1;
+                    while (c * 2 <= tpw) c *= 2;
+                    a.pd.tpw = std::max(a.pd.tpw, c);
                 }
                 lds_need = std::max(lds_need, sl * (size_t)a.pd.tpw);
                 table.push_back(a);

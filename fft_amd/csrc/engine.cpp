@@ -238,15 +238,6 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     return true;
 }
 
-static int pow2_part(size_t x) {
-    int p = 1;
-    while (x % 2 == 0 && p < (1 << 20)) {
-        x /= 2;
-        p *= 2;
-    }
-    return p;
-}
-
 Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind), prec_(prec) {
     if (kind != kC2C && kind != kR2C && kind != kR2CHalf)
         throw Error(SFFT_ERR_INVALID_ARG, "unknown transform kind");
@@ -302,8 +293,10 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
             if (h && h->split > 1 && n1 != h->split) continue;
             const size_t n2 = cn_ / n1;
             PassPlan p1, p2;
-            if (!layout_columns((int)n1, prec, pow2_part(n2), nullptr, p1)) continue;
-            if (!layout_columns((int)n2, prec, pow2_part(n1), nullptr, p2)) continue;
+            if (!layout_columns((int)n1, prec, (int)std::min<size_t>(n2, 16), nullptr, p1))
+                continue;
+            if (!layout_columns((int)n2, prec, (int)std::min<size_t>(n1, 16), nullptr, p2))
+                continue;
             if (p1.pd.nstages < 2) continue;
             // prefer wide column tiles (coalescing), then balance
             const double score = -std::log2((double)std::min(p1.pd.tpw, p2.pd.tpw)) * 4.0 +
@@ -316,8 +309,8 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         if (!best_n1) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "no two-pass split");
         const size_t n1 = best_n1, n2 = cn_ / n1;
         PassPlan p1, p2;
-        layout_columns((int)n1, prec, pow2_part(n2), nullptr, p1);
-        layout_columns((int)n2, prec, pow2_part(n1), nullptr, p2);
+        layout_columns((int)n1, prec, (int)std::min<size_t>(n2, 16), nullptr, p1);
+        layout_columns((int)n2, prec, (int)std::min<size_t>(n1, 16), nullptr, p2);
         p1.slots_per_transform = (int)n2;
         p1.in_slot_stride = 1;
         p1.in_elem_stride = (int64_t)n2;

@@ -38,7 +38,11 @@ __global__ void __launch_bounds__(kPersistentBlock)
                 blk.run();
             }
             if (p + 1 < npasses || (wu.npasses & kUnitRealPost)) {
-                __threadfence();  // this pass' stores visible before the next loads
+                // The next pass is read by this same workgroup (same CU), and no
+                // other workgroup touches this unit's output: a workgroup-scope
+                // fence (stores complete) suffices.  A device-scope fence here
+                // would write back the whole L2 of the XCD for every unit.
+                __threadfence_block();
                 __syncthreads();
             }
         }

@@ -102,16 +102,46 @@ struct Block {
     int slot0;
     const int64_t *gather_in = nullptr;
     const int64_t *gather_out = nullptr;
+    // Uniform descriptor fields, read once: the descriptor may live in global
+    // memory (persistent launches), where reading it inside the element loops
+    // would be repeated after every store the compiler cannot disambiguate.
+    int n_, tpt_, tpw_, lds_stride_, nstages_, in_col_, out_col_;
+    int conj_in_, conj_out_, mode_, half_shift_, post_tw_, post_tw_shift_, real_h_;
+    IoDesc iin_, iout_;
+    const C2<T> *tw_, *real_tw_, *rot_;
+    const C2<double> *ptw_fine_, *ptw_coarse_;
 
     __device__ Block(const PassArgs<T> &args, const C2<T> *in_, C2<T> *out_, int total, C2<T> *l,
                      int t, int s0)
-        : a(args), in(in_), out(out_), total_slots(total), lds(l), tid(t), slot0(s0) {}
+        : a(args), in(in_), out(out_), total_slots(total), lds(l), tid(t), slot0(s0) {
+        n_ = args.pd.n;
+        tpt_ = args.pd.tpt;
+        tpw_ = args.pd.tpw;
+        lds_stride_ = args.pd.lds_stride;
+        nstages_ = args.pd.nstages;
+        in_col_ = args.pd.in_col;
+        out_col_ = args.pd.out_col;
+        conj_in_ = args.conj_in;
+        conj_out_ = args.conj_out;
+        mode_ = args.mode;
+        half_shift_ = args.half_shift;
+        post_tw_ = args.post_tw;
+        post_tw_shift_ = args.post_tw_shift;
+        real_h_ = args.real_h;
+        iin_ = args.iin;
+        iout_ = args.iout;
+        tw_ = args.tw;
+        real_tw_ = args.real_tw;
+        rot_ = args.rot;
+        ptw_fine_ = args.ptw_fine;
+        ptw_coarse_ = args.ptw_coarse;
+    }
 
     __device__ __forceinline__ int64_t ibase(int g) const {
-        return gather_in ? gather_in[g] : gbase(a.iin, g);
+        return gather_in ? gather_in[g] : gbase(iin_, g);
     }
     __device__ __forceinline__ int64_t obase(int g) const {
-        return gather_out ? gather_out[g] : gbase(a.iout, g);
+        return gather_out ? gather_out[g] : gbase(iout_, g);
     }
 
     // transform base and in-transform offset of slot g's element 0
@@ -136,70 +166,69 @@ struct Block {
 
     __device__ __forceinline__ C2<T> post_twiddle(int g, int e) const {
         // W_N^{c e}, c = column index of slot g, via a two-level FP64 table.
-        const uint32_t c = (uint32_t)g - fdiv((uint32_t)g, a.iout.per_batch) * a.iout.per_batch.d;
+        const uint32_t c = (uint32_t)g - fdiv((uint32_t)g, iout_.per_batch) * iout_.per_batch.d;
         // c < N2 and e < N1, so c e < N: no reduction needed.
         const uint32_t m = c * (uint32_t)e;
-        const C2<double> f = a.ptw_fine[m & ((1u << a.post_tw_shift) - 1u)];
-        const C2<double> h = a.ptw_coarse[m >> a.post_tw_shift];
+        const C2<double> f = ptw_fine_[m & ((1u << post_tw_shift_) - 1u)];
+        const C2<double> h = ptw_coarse_[m >> post_tw_shift_];
         return {(T)fmadd(f.x, h.x, -(f.y * h.y)), (T)fmadd(f.x, h.y, f.y * h.x)};
     }
 
     // One Stockham stage of radix R.
     template <int R>
     __device__ __forceinline__ void stage(int s, bool src_global, bool dst_global, bool col_map) {
-        const PassDesc &pd = a.pd;
-        const StageDesc st = pd.st[s];
+        const StageDesc st = a.pd.st[s];
         constexpr int B = (EPT / R) > 0 ? (EPT / R) : 1;
-        const int n = pd.n;
+        const int n = n_;
         const int nb = n / R;
         int slot, lid;
         if (col_map) {
-            slot = tid % pd.tpw;
-            lid = tid / pd.tpw;
+            slot = tid % tpw_;
+            lid = tid / tpw_;
         } else {
-            slot = tid / pd.tpt;
-            lid = tid % pd.tpt;
+            slot = tid / tpt_;
+            lid = tid % tpt_;
         }
         const int g = slot0 + slot;
-        const bool active = (slot < pd.tpw) && (lid < pd.tpt) && (g < total_slots);
-        C2<T> *ls = lds + slot * pd.lds_stride;
+        const bool active = (slot < tpw_) && (lid < tpt_) && (g < total_slots);
+        C2<T> *ls = lds + slot * lds_stride_;
 
         C2<T> v[B][R];
         // ---- load
-        if (src_global && a.mode == kModeC2RLoad) {
+        if (src_global && mode_ == kModeC2RLoad) {
             // two-pass inverse real: pre-process each element from its pair
             int64_t tb = 0, cofs = 0;
-            if (active) split_slot(a.iin, g, tb, cofs);
-            const int64_t es = a.iin.elem_stride;
+            if (active) split_slot(iin_, g, tb, cofs);
+            const int64_t es = iin_.elem_stride;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const int j = lid + b * pd.tpt;
+                const int j = lid + b * tpt_;
                 if (active && j < nb) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int m = (int)(cofs + (int64_t)(j + r * nb) * es);
-                        v[b][r] = cconj(real_pre_bin(in + tb, a.real_tw, m, a.real_h,
-                                                     a.half_shift != 0));
+                        v[b][r] = cconj(real_pre_bin(in + tb, real_tw_, m, real_h_,
+                                                     half_shift_ != 0));
                     }
                 }
             }
         } else if (src_global) {
             const int64_t base = active ? ibase(g) : 0;
-            const int64_t es = a.iin.elem_stride;
+            const int64_t es = iin_.elem_stride;
             int64_t tb = 0, cofs = 0;
-            if (active && a.mode == kModeR2CLoad) split_slot(a.iin, g, tb, cofs);
+            if (active && mode_ == kModeR2CLoad) split_slot(iin_, g, tb, cofs);
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const int j = lid + b * pd.tpt;
+                const int j = lid + b * tpt_;
                 if (active && j < nb) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int e = j + r * nb;
                         C2<T> x = in[base + (int64_t)e * es];
-                        if (a.conj_in) x.y = -x.y;
-                        if (a.mode == kModeR2C && a.half_shift) x = cmul(x, a.rot[e]);
-                        if (a.mode == kModeR2CLoad && a.half_shift)
-                            x = cmul(x, a.rot[cofs + (int64_t)e * es]);
+                        if (conj_in_) x.y = -x.y;
+                        if (mode_ == kModeR2C && half_shift_) x = cmul(x, rot_[e]);
+                        if (mode_ == kModeR2CLoad && half_shift_)
+                            x = cmul(x, rot_[cofs + (int64_t)e * es]);
                         v[b][r] = x;
                     }
                 }
@@ -207,7 +236,7 @@ struct Block {
         } else {
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const int j = lid + b * pd.tpt;
+                const int j = lid + b * tpt_;
                 if (active && j < nb) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) v[b][r] = ls[lds_idx(j + r * nb)];
@@ -219,12 +248,12 @@ struct Block {
         const int ns = st.ns;
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            const int j = lid + b * pd.tpt;
+            const int j = lid + b * tpt_;
             if (active && j < nb) {
                 if (ns > 1) {
                     const uint32_t q = fdiv((uint32_t)j, st.div);
                     const int k = j - (int)q * ns;
-                    const C2<T> *w = a.tw + st.tw_off + k * tw_group(R);
+                    const C2<T> *w = tw_ + st.tw_off + k * tw_group(R);
 #pragma unroll
                     for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r - 1]);
                 }
@@ -234,12 +263,12 @@ struct Block {
         // ---- store
         if (dst_global) {
             const int64_t base = active ? obase(g) : 0;
-            const int64_t es = a.iout.elem_stride;
+            const int64_t es = iout_.elem_stride;
             int64_t tbo = 0, cofo = 0;
-            if (active && a.mode == kModeC2RStore) split_slot(a.iout, g, tbo, cofo);
+            if (active && mode_ == kModeC2RStore) split_slot(iout_, g, tbo, cofo);
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const int j = lid + b * pd.tpt;
+                const int j = lid + b * tpt_;
                 if (active && j < nb) {
                     const uint32_t q = fdiv((uint32_t)j, st.div);
                     const int k = j - (int)q * ns;
@@ -248,11 +277,11 @@ struct Block {
                     for (int r = 0; r < R; ++r) {
                         const int e = o + r * ns;
                         C2<T> y = v[b][r];
-                        if (a.conj_out) y.y = -y.y;
-                        if (a.mode == kModeC2R && a.half_shift) y = cmulc(y, a.rot[e]);
-                        if (a.mode == kModeC2RStore && a.half_shift)
-                            y = cmulc(y, a.rot[cofo + (int64_t)e * es]);
-                        if (a.post_tw) y = cmul(y, post_twiddle(g, e));
+                        if (conj_out_) y.y = -y.y;
+                        if (mode_ == kModeC2R && half_shift_) y = cmulc(y, rot_[e]);
+                        if (mode_ == kModeC2RStore && half_shift_)
+                            y = cmulc(y, rot_[cofo + (int64_t)e * es]);
+                        if (post_tw_) y = cmul(y, post_twiddle(g, e));
                         out[base + (int64_t)e * es] = y;
                     }
                 }
@@ -260,7 +289,7 @@ struct Block {
         } else {
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const int j = lid + b * pd.tpt;
+                const int j = lid + b * tpt_;
                 if (active && j < nb) {
                     const uint32_t q = fdiv((uint32_t)j, st.div);
                     const int k = j - (int)q * ns;
@@ -298,26 +327,25 @@ struct Block {
     // Real inverse pre-process (pairs i, h - i) from HBM into LDS, writing
     // conj(.) so that the forward stages compute the inverse.
     __device__ __forceinline__ void real_pre() {
-        const PassDesc &pd = a.pd;
-        const int slot = tid / pd.tpt, lid = tid % pd.tpt;
+        const int slot = tid / tpt_, lid = tid % tpt_;
         const int g = slot0 + slot;
-        const bool active = (slot < pd.tpw) && (g < total_slots);
-        const int h = pd.n;
-        const bool mod = a.half_shift != 0;
-        C2<T> *ls = lds + slot * pd.lds_stride;
+        const bool active = (slot < tpw_) && (g < total_slots);
+        const int h = n_;
+        const bool mod = half_shift_ != 0;
+        C2<T> *ls = lds + slot * lds_stride_;
         if (active) {
             const C2<T> *x = in + ibase(g);
             if (!mod && lid == 0) {
                 const C2<T> v = x[0];
                 ls[0] = {v.x + v.y, -(v.x - v.y)};
             }
-            for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += pd.tpt) {
+            for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += tpt_) {
                 if (real_pair_skipped(i, h, mod)) continue;
                 const int ci = mod ? (h - 1 - i) : (h - i);
                 const C2<T> v = x[i], v2 = x[ci];
                 const C2<T> odd = {v.x + v2.x, v.y - v2.y};
                 const C2<T> erot = {v.x - v2.x, v.y + v2.y};
-                const C2<T> ev = cmulc(erot, a.real_tw[i]);
+                const C2<T> ev = cmulc(erot, real_tw_[i]);
                 const C2<T> p = cadd(odd, ev);
                 const C2<T> m = csub(odd, ev);
                 ls[lds_idx(i)] = cconj(p);
@@ -329,13 +357,12 @@ struct Block {
 
     // Real forward post-process from the complex spectrum in LDS to HBM.
     __device__ __forceinline__ void real_post() {
-        const PassDesc &pd = a.pd;
-        const int slot = tid / pd.tpt, lid = tid % pd.tpt;
+        const int slot = tid / tpt_, lid = tid % tpt_;
         const int g = slot0 + slot;
-        const bool active = (slot < pd.tpw) && (g < total_slots);
-        const int h = pd.n;
-        const bool mod = a.half_shift != 0;
-        const C2<T> *ls = lds + slot * pd.lds_stride;
+        const bool active = (slot < tpw_) && (g < total_slots);
+        const int h = n_;
+        const bool mod = half_shift_ != 0;
+        const C2<T> *ls = lds + slot * lds_stride_;
         if (!active) return;
         C2<T> *y = out + obase(g);
         if (!mod && lid == 0) {
@@ -343,13 +370,13 @@ struct Block {
             y[0] = {z.x + z.y, z.x - z.y};
         }
         const T half = (T)0.5;
-        for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += pd.tpt) {
+        for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += tpt_) {
             if (real_pair_skipped(i, h, mod)) continue;
             const int ci = mod ? (h - 1 - i) : (h - i);
             const C2<T> zi = ls[lds_idx(i)], zc = ls[lds_idx(ci)];
             const C2<T> odd = {(zi.x + zc.x) * half, (zi.y - zc.y) * half};
             const C2<T> ev = {(zi.x - zc.x) * half, (zi.y + zc.y) * half};
-            const C2<T> e = cmul(ev, a.real_tw[i]);
+            const C2<T> e = cmul(ev, real_tw_[i]);
             y[i] = cadd(odd, e);
             y[ci] = cconj(csub(odd, e));
         }
@@ -362,16 +389,15 @@ struct Block {
     }
 
     __device__ __forceinline__ void run() {
-        const PassDesc &pd = a.pd;
-        const int last = pd.nstages - 1;
-        if (a.mode == kModeC2R) real_pre();
+        const int last = nstages_ - 1;
+        if (mode_ == kModeC2R) real_pre();
         for (int s = 0; s <= last; ++s) {
-            const bool src_g = (s == 0) && (a.mode != kModeC2R);
-            const bool dst_g = (s == last) && (a.mode != kModeR2C);
-            const bool col = (s == 0) ? (pd.in_col != 0) : (s == last) ? (pd.out_col != 0) : false;
+            const bool src_g = (s == 0) && (mode_ != kModeC2R);
+            const bool dst_g = (s == last) && (mode_ != kModeR2C);
+            const bool col = (s == 0) ? (in_col_ != 0) : (s == last) ? (out_col_ != 0) : false;
             dispatch(s, src_g, dst_g, col);
         }
-        if (a.mode == kModeR2C) real_post_sync();
+        if (mode_ == kModeR2C) real_post_sync();
     }
 };
 
