@@ -43,10 +43,32 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain-sample", type=int, default=256,
-                    help="transforms per size timed on the plain per-call path")
+                    help="transforms per size timed on the plain per-call path (0: skip)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the uneven-batch and serving measurements")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only time a 1 GiB device-to-device copy (the reference kernel for "
+                         "scaling HBM byte counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(torch):
+    """Three 1 GiB device copies: 1 GiB read + 1 GiB written per launch."""
+    nbytes = 1 << 30
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 3
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "ms": round(ms, 4), "GB/s": round(2 * nbytes / ms / 1e6, 1)}))
 
 
 class Launch:
@@ -273,6 +295,9 @@ def main():
     from fft_amd import _lib as L
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     launches, bufs = build_workload(torch, fft_amd, L)
     total_ms, wall = time_steps(torch, launches, args.steps, args.warmup)
     step_bytes = sum(ln.alg_bytes for ln in launches)
@@ -293,7 +318,8 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get(dom.name)
 
-    plain_gbs, plain_per = plain_per_call(torch, fft_amd, L, bufs, args.plain_sample)
+    plain_gbs, plain_per = (plain_per_call(torch, fft_amd, L, bufs, args.plain_sample)
+                            if args.plain_sample > 0 else (None, None))
 
     result = {
         "metric": METRIC,
@@ -318,10 +344,11 @@ def main():
         },
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "per_launch": per,
-        "plain_per_call": {"value": round(plain_gbs, 2), "unit": "GB/s",
-                           "sample": f"{args.plain_sample} single-transform calls per size "
-                                     "and direction", "per_size": plain_per},
-        "speedup_vs_plain": round(value / plain_gbs, 1),
+        "plain_per_call": None if plain_gbs is None else {
+            "value": round(plain_gbs, 2), "unit": "GB/s",
+            "sample": f"{args.plain_sample} single-transform calls per size and direction",
+            "per_size": plain_per},
+        "speedup_vs_plain": None if plain_gbs is None else round(value / plain_gbs, 1),
         "roofline": {"bound": "hbm", "kernel": dom.name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
