@@ -136,6 +136,7 @@ int sfft_plan_get_info(sfft_plan plan, sfft_plan_info *info) {
         info->transforms_per_block = pd.tpw;
         info->workspace_bytes = 0;
         info->table_bytes = p.table_bytes();
+        info->mfma_stages = p.mfma_stages();
         return SFFT_OK;
     });
 }

@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 namespace sfft {
 
@@ -135,6 +136,7 @@ static bool only_small_primes(size_t n) {
 bool supported_length(size_t n) { return n >= 1 && n <= (1ull << 28) && only_small_primes(n); }
 
 static size_t elem_bytes(int prec) { return prec == kF64 ? 16 : 8; }
+static int max_block(int prec) { return prec >= kF16 ? kMaxBlock16 : 1024; }
 
 // Dynamic LDS a layout may use: the device limit less room for the kernels'
 // static __shared__ variables.
@@ -175,8 +177,9 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
             for (int r : h->radices) prod *= r;
             if (prod != n) return false;
             radices = h->radices;
-            if (threads_for(n, radices, e) > 1024) continue;
-        } else if (!choose_radices(n, e, radices, prec == kF64 ? 16 : 25)) {
+            if (threads_for(n, radices, e) > max_block(prec)) continue;
+        } else if (!choose_radices(n, e, radices, prec == kF64 ? 16 : 25) ||
+                   threads_for(n, radices, e) > max_block(prec)) {
             continue;
         }
         ept = e;
@@ -190,7 +193,8 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
     if (slot_lds > lds_budget()) return false;
     int tpw = std::max(1, 256 / tpt);
     if (h && h->tpw) tpw = h->tpw;
-    if (tpw * tpt > 1024) tpw = std::max(1, 1024 / tpt);
+    if (tpw * tpt > max_block(prec)) tpw = std::max(1, max_block(prec) / tpt);
+    if (tpt > max_block(prec)) return false;
     if (slot_lds) tpw = std::min<int>(tpw, (int)(lds_budget() / slot_lds));
     if (tpw < 1) return false;
     pp = PassPlan{};
@@ -212,7 +216,8 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     int ept = 0;
     for (int e : {kEptSmall, kEptLarge}) {
         if (prec == kF64 && e > kEptSmall) continue;
-        if (choose_radices(n, e, radices, prec == kF64 ? 16 : 25)) {
+        if (choose_radices(n, e, radices, prec == kF64 ? 16 : 25) &&
+            threads_for(n, radices, e) <= max_block(prec)) {
             ept = e;
             break;
         }
@@ -224,7 +229,7 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     const size_t slot_lds = (size_t)stride * eb;
     int cols = std::min(max_cols, 16);
     cols = std::min<int>(cols, (int)(lds_budget() / slot_lds));
-    cols = std::min(cols, 1024 / tpt);
+    cols = std::min(cols, max_block(prec) / tpt);
     if (cols < 1) return false;
     cols = floor_pow2(cols);
     pp = PassPlan{};
@@ -238,10 +243,26 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     return true;
 }
 
+// 16-bit storage: run the pass' radix-16 stages on MFMA when every wave of
+// the block takes whole groups of 32 butterflies (stockham.h stage16_mfma).
+static void enable_mfma(PassPlan &pp) {
+    const int n = pp.pd.n;
+    bool has16 = false;
+    for (int s = 0; s < pp.pd.nstages; ++s) has16 |= pp.pd.st[s].radix == 16;
+    if (!has16 || n % 16) return;
+    const int nf = pp.pd.tpw * (n / 16);
+    const int waves = (int)(pp.block / 64);
+    pp.pd.mfma = (pp.block % 64 == 0) && (nf % 32 == 0) && (nf / 32 <= waves * (pp.ept / 8));
+}
+
 Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind), prec_(prec) {
     if (kind != kC2C && kind != kR2C && kind != kR2CHalf)
         throw Error(SFFT_ERR_INVALID_ARG, "unknown transform kind");
-    if (prec != kF32 && prec != kF64) throw Error(SFFT_ERR_INVALID_ARG, "unknown precision");
+    if (prec != kF32 && prec != kF64 && prec != kF16 && prec != kBF16)
+        throw Error(SFFT_ERR_INVALID_ARG, "unknown precision");
+    if (prec >= kF16 && kind != kC2C)
+        throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
+                    "16-bit storage runs complex transforms only (real kinds: F32 or F64)");
     if (n == 0) throw Error(SFFT_ERR_INVALID_ARG, "size 0");
     if (kind != kC2C && (n % 2))
         throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "real transforms need an even size");
@@ -330,10 +351,21 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         passes_.push_back(p1);
         passes_.push_back(p2);
     }
+    if (prec >= kF16) {
+        for (auto &pp : passes_) enable_mfma(pp);
+    }
     if (prec == kF64)
         build_tables<double>();
     else
         build_tables<float>();
+}
+
+int Plan::mfma_stages() const {
+    int c = 0;
+    for (const auto &pp : passes_)
+        if (pp.pd.mfma)
+            for (int s = 0; s < pp.pd.nstages; ++s) c += pp.pd.st[s].radix == 16;
+    return c;
 }
 
 Plan::~Plan() {
@@ -409,12 +441,12 @@ size_t Plan::default_out_dist(int inverse) const {
 }
 
 size_t Plan::in_bytes(size_t batch, int inverse) const {
-    const size_t rb = prec_ == kF64 ? 8 : 4;
+    const size_t rb = real_bytes(prec_);
     if (kind_ == kC2C || inverse) return batch * cn_ * 2 * rb;
     return batch * n_ * rb;
 }
 size_t Plan::out_bytes(size_t batch, int inverse) const {
-    const size_t rb = prec_ == kF64 ? 8 : 4;
+    const size_t rb = real_bytes(prec_);
     if (kind_ == kC2C || !inverse) return batch * cn_ * 2 * rb;
     return batch * n_ * rb;
 }
@@ -494,6 +526,14 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         a.out = (C2<T> *)out;
         a.total_slots = (int32_t)slots;
         const unsigned grid = (unsigned)((slots + pp.pd.tpw - 1) / pp.pd.tpw);
+        if constexpr (std::is_same<T, float>::value) {
+            if (prec_ >= kF16) {
+                check_hip(launch_pass_16(a, prec_ == kF16 ? kStorageF16 : kStorageBF16, pp.ept,
+                                         grid, pp.block, pp.lds_bytes, stream),
+                          "pass launch");
+                continue;
+            }
+        }
         check_hip(launch_pass<T>(a, pp.ept, grid, pp.block, pp.lds_bytes, stream), "pass launch");
     }
     // two-pass forward real: pairwise post-process in place on the output
@@ -541,7 +581,7 @@ void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, s
     if (batch == 0) return;
     if (!in || !out) throw Error(SFFT_ERR_INVALID_ARG, "null buffer");
     // overlap check on the byte ranges actually touched
-    const size_t rb = prec_ == kF64 ? 8 : 4;
+    const size_t rb = real_bytes(prec_);
     const size_t in_unit = (kind_ == kC2C || inverse) ? 2 * rb : rb;
     const size_t out_unit = (kind_ == kC2C || !inverse) ? 2 * rb : rb;
     const size_t in_len = ((batch - 1) * in_dist + default_in_dist(inverse)) * in_unit;

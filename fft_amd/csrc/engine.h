@@ -18,7 +18,11 @@
 namespace sfft {
 
 enum Kind : int { kC2C = 0, kR2C = 1, kR2CHalf = 2 };
-enum Prec : int { kF32 = 0, kF64 = 1 };
+// kF16 / kBF16: float arithmetic, 16-bit HBM storage (complex-to-complex only).
+enum Prec : int { kF32 = 0, kF64 = 1, kF16 = 2, kBF16 = 3 };
+
+// Bytes of one real component in HBM.
+inline size_t real_bytes(int prec) { return prec == kF64 ? 8 : (prec >= kF16 ? 2 : 4); }
 
 // Error with an ABI status code.
 struct Error : std::runtime_error {
@@ -79,6 +83,7 @@ class Plan {
     size_t complex_len() const { return cn_; }
     const std::vector<PassPlan> &passes() const { return passes_; }
     size_t table_bytes() const { return table_bytes_; }
+    int mfma_stages() const;
 
     // in/out: device pointers; dists in elements of the in/out type
     // (complex for C2C, real/complex for the real kinds).

@@ -1,0 +1,47 @@
+// half.hip -- pass kernel instances for 16-bit HBM storage (IEEE half,
+// bfloat16) with float arithmetic and radix-16 stages on MFMA.
+#include "kernels.h"
+
+namespace sfft {
+
+template <int EPT, typename S>
+__global__ void __launch_bounds__(kMaxBlock16) k_pass16(PassArgs<float> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<float> *lds = reinterpret_cast<C2<float> *>(smem_raw);
+    Block<float, EPT, S> blk(a, a.in, a.out, a.total_slots, lds, (int)threadIdx.x,
+                             (int)xcd_tile(blockIdx.x, gridDim.x) * a.pd.tpw);
+    blk.run();
+}
+
+template <int EPT, typename S>
+static hipError_t launch16(const PassArgs<float> &a, unsigned grid, unsigned block,
+                           size_t lds_bytes, hipStream_t stream) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16<EPT, S>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024 - kLdsReserve);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return e;
+        }
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_pass16<EPT, S>), dim3(grid), dim3(block), lds_bytes, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_16(const PassArgs<float> &a, int storage, int ept, unsigned grid,
+                          unsigned block, size_t lds_bytes, hipStream_t stream) {
+    if (grid == 0) return hipSuccess;
+    const bool small = ept <= kEptSmall;
+    if (storage == kStorageF16)
+        return small ? launch16<kEptSmall, StoreF16>(a, grid, block, lds_bytes, stream)
+                     : launch16<kEptLarge, StoreF16>(a, grid, block, lds_bytes, stream);
+    if (storage == kStorageBF16)
+        return small ? launch16<kEptSmall, StoreBF16>(a, grid, block, lds_bytes, stream)
+                     : launch16<kEptLarge, StoreBF16>(a, grid, block, lds_bytes, stream);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sfft

@@ -15,6 +15,16 @@ template <typename T>
 hipError_t launch_pass(const PassArgs<T> &a, int ept, unsigned grid, unsigned block,
                        size_t lds_bytes, hipStream_t stream);
 
+// 16-bit HBM storage with float arithmetic (half.hip); a.in / a.out address
+// 4-byte complex values.
+constexpr int kStorageF16 = 1;
+constexpr int kStorageBF16 = 2;
+// Block size limit of the 16-bit storage instances (room for the MFMA
+// stage's operands without spilling).
+constexpr int kMaxBlock16 = 512;
+hipError_t launch_pass_16(const PassArgs<float> &a, int storage, int ept, unsigned grid,
+                          unsigned block, size_t lds_bytes, hipStream_t stream);
+
 // In-place forward real post-process of batch spectra of length h (distance
 // dist complex values) after a two-pass complex FFT.
 template <typename T>

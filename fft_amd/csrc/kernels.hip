@@ -11,7 +11,7 @@ __global__ void __launch_bounds__(1024) k_pass(PassArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
     Block<T, EPT> blk(a, a.in, a.out, a.total_slots, lds, (int)threadIdx.x,
-                      (int)blockIdx.x * a.pd.tpw);
+                      (int)xcd_tile(blockIdx.x, gridDim.x) * a.pd.tpw);
     blk.run();
 }
 

@@ -58,7 +58,7 @@ struct PassDesc {
     int32_t lds_stride;  // padded LDS elements per slot
     int32_t in_col;      // stage 0 mapping: lanes along slots (strided elements)
     int32_t out_col;     // last stage mapping
-    int32_t pad;
+    int32_t mfma;        // 16-bit storage: radix-16 stages as MFMA contractions
     StageDesc st[kMaxStages];
 };
 

@@ -9,6 +9,7 @@
 
 #include "dft.h"
 #include "plan_desc.h"
+#include "storage.h"
 
 namespace sfft {
 
@@ -18,6 +19,17 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
 }
 
 __device__ __forceinline__ int lds_idx(int e) { return e + (e >> kLdsPadShift); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Tile index of block b so that every XCD runs a contiguous range of
+// tiles: neighbouring column tiles, which share HBM lines when a tile is
+// narrower than a line, then meet in the same L2.  A bijection on [0, nb).
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+    const unsigned q = nb / kXcds, r = nb % kXcds;
+    const unsigned x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
 
 // Real-FFT fusion modes.
 enum : int32_t {
@@ -91,8 +103,11 @@ __device__ __forceinline__ C2<T> real_pre_bin(const C2<T> *x, const C2<T> *tmi, 
 // (persistent launches); in, out and total_slots are per launch / per item.
 // Optional gather lists (persistent launches) give each slot its own input /
 // output offset in elements instead of the strided address map.
-template <typename T, int EPT>
+// S is the HBM storage format of in / out (storage.h); with a 16-bit format
+// the pointers address 4-byte values and only complex-to-complex modes run.
+template <typename T, int EPT, typename S = StoreNative>
 struct Block {
+    using St = Storage<T, S>;
     const PassArgs<T> &a;
     const C2<T> *in;
     C2<T> *out;
@@ -106,7 +121,7 @@ struct Block {
     // memory (persistent launches), where reading it inside the element loops
     // would be repeated after every store the compiler cannot disambiguate.
     int n_, tpt_, tpw_, lds_stride_, nstages_, in_col_, out_col_;
-    int conj_in_, conj_out_, mode_, half_shift_, post_tw_, post_tw_shift_, real_h_;
+    int conj_in_, conj_out_, mode_, half_shift_, post_tw_, post_tw_shift_, real_h_, mfma_;
     IoDesc iin_, iout_;
     const C2<T> *tw_, *real_tw_, *rot_;
     const C2<double> *ptw_fine_, *ptw_coarse_;
@@ -128,6 +143,7 @@ struct Block {
         post_tw_ = args.post_tw;
         post_tw_shift_ = args.post_tw_shift;
         real_h_ = args.real_h;
+        mfma_ = args.pd.mfma;
         iin_ = args.iin;
         iout_ = args.iout;
         tw_ = args.tw;
@@ -177,6 +193,12 @@ struct Block {
     // One Stockham stage of radix R.
     template <int R>
     __device__ __forceinline__ void stage(int s, bool src_global, bool dst_global, bool col_map) {
+        if constexpr (R == 16 && !St::native) {
+            if (mfma_) {
+                stage16_mfma(s, src_global, dst_global, col_map);
+                return;
+            }
+        }
         const StageDesc st = a.pd.st[s];
         constexpr int B = (EPT / R) > 0 ? (EPT / R) : 1;
         const int n = n_;
@@ -195,7 +217,7 @@ struct Block {
 
         C2<T> v[B][R];
         // ---- load
-        if (src_global && mode_ == kModeC2RLoad) {
+        if (St::native && src_global && mode_ == kModeC2RLoad) {
             // two-pass inverse real: pre-process each element from its pair
             int64_t tb = 0, cofs = 0;
             if (active) split_slot(iin_, g, tb, cofs);
@@ -207,8 +229,9 @@ struct Block {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int m = (int)(cofs + (int64_t)(j + r * nb) * es);
-                        v[b][r] = cconj(real_pre_bin(in + tb, real_tw_, m, real_h_,
-                                                     half_shift_ != 0));
+                        if constexpr (St::native)
+                            v[b][r] = cconj(real_pre_bin(in + tb, real_tw_, m, real_h_,
+                                                         half_shift_ != 0));
                     }
                 }
             }
@@ -224,7 +247,7 @@ struct Block {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const int e = j + r * nb;
-                        C2<T> x = in[base + (int64_t)e * es];
+                        C2<T> x = St::load(in, base + (int64_t)e * es);
                         if (conj_in_) x.y = -x.y;
                         if (mode_ == kModeR2C && half_shift_) x = cmul(x, rot_[e]);
                         if (mode_ == kModeR2CLoad && half_shift_)
@@ -282,7 +305,7 @@ struct Block {
                         if (mode_ == kModeC2RStore && half_shift_)
                             y = cmulc(y, rot_[cofo + (int64_t)e * es]);
                         if (post_tw_) y = cmul(y, post_twiddle(g, e));
-                        out[base + (int64_t)e * es] = y;
+                        St::store(out, base + (int64_t)e * es, y);
                     }
                 }
             }
@@ -299,6 +322,101 @@ struct Block {
                 }
             }
             __syncthreads();
+        }
+    }
+
+    // Radix-16 stage on the matrix cores (16-bit storage).  The tile's
+    // tpw * n/16 butterflies are flattened (slot-major, or slot-minor for
+    // column-strided HBM access) and taken 32 at a time by a wave: lane l
+    // holds points mfma_point(i, l / 32) of butterfly l % 32 of its group.
+    // The host enables it only when blockDim is a multiple of 64, the
+    // butterfly count a multiple of 32 and at most EPT / 8 groups per wave.
+    __device__ __forceinline__ void stage16_mfma(int s, bool src_global, bool dst_global,
+                                                 bool col_map) {
+        if constexpr (!St::native) {
+            constexpr int GM = EPT / 8;
+            const StageDesc st = a.pd.st[s];
+            const int nb = n_ / 16, ns = st.ns;
+            const int lane = tid & 63, wv = tid >> 6, nw = (int)(blockDim.x >> 6);
+            const int col = lane & 31, hh = lane >> 5;
+            const int ngroups = (tpw_ * nb) >> 5;
+            C2<T> v[GM][8];
+            int gslot[GM], gj[GM];
+#pragma unroll
+            for (int q = 0; q < GM; ++q) {
+                const int gi = wv + q * nw;
+                gslot[q] = 0;
+                gj[q] = 0;
+                if (gi < ngroups) {  // wave-uniform
+                    const int f = gi * 32 + col;
+                    const int slot = col_map ? f % tpw_ : f / nb;
+                    const int j = col_map ? f / tpw_ : f % nb;
+                    gslot[q] = slot;
+                    gj[q] = j;
+                    const int g = slot0 + slot;
+                    if (src_global) {
+                        const bool act = g < total_slots;
+                        const int64_t base = act ? ibase(g) : 0;
+                        const int64_t es = iin_.elem_stride;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            C2<T> x = {(T)0, (T)0};
+                            if (act) x = St::load(in, base + (int64_t)(j + mfma_point(i, hh) * nb) * es);
+                            if (conj_in_) x.y = -x.y;
+                            v[q][i] = x;
+                        }
+                    } else {
+                        const C2<T> *ls = lds + slot * lds_stride_;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) v[q][i] = ls[lds_idx(j + mfma_point(i, hh) * nb)];
+                    }
+                }
+            }
+            if (!src_global) __syncthreads();
+            const Dft16Operands<S> dm(lane);
+#pragma unroll
+            for (int q = 0; q < GM; ++q) {
+                if (wv + q * nw < ngroups) {
+                    if (ns > 1) {
+                        const int k = gj[q] % ns;
+                        const C2<T> *w = tw_ + st.tw_off + k * tw_group(16);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const int r = mfma_point(i, hh);
+                            if (r) v[q][i] = cmul(v[q][i], w[r - 1]);
+                        }
+                    }
+                    dft16_mfma<S>(v[q], dm);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GM; ++q) {
+                if (wv + q * nw < ngroups) {
+                    const int j = gj[q], slot = gslot[q];
+                    const int qq = j / ns, k = j - qq * ns;
+                    const int o = qq * ns * 16 + k;
+                    const int g = slot0 + slot;
+                    if (dst_global) {
+                        if (g < total_slots) {
+                            const int64_t base = obase(g);
+                            const int64_t es = iout_.elem_stride;
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                const int e = o + mfma_point(i, hh) * ns;
+                                C2<T> y = v[q][i];
+                                if (conj_out_) y.y = -y.y;
+                                if (post_tw_) y = cmul(y, post_twiddle(g, e));
+                                St::store(out, base + (int64_t)e * es, y);
+                            }
+                        }
+                    } else {
+                        C2<T> *ls = lds + slot * lds_stride_;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) ls[lds_idx(o + mfma_point(i, hh) * ns)] = v[q][i];
+                    }
+                }
+            }
+            if (!dst_global) __syncthreads();
         }
     }
 
@@ -327,6 +445,7 @@ struct Block {
     // Real inverse pre-process (pairs i, h - i) from HBM into LDS, writing
     // conj(.) so that the forward stages compute the inverse.
     __device__ __forceinline__ void real_pre() {
+        if constexpr (!St::native) return;
         const int slot = tid / tpt_, lid = tid % tpt_;
         const int g = slot0 + slot;
         const bool active = (slot < tpw_) && (g < total_slots);
@@ -357,6 +476,7 @@ struct Block {
 
     // Real forward post-process from the complex spectrum in LDS to HBM.
     __device__ __forceinline__ void real_post() {
+        if constexpr (!St::native) return;
         const int slot = tid / tpt_, lid = tid % tpt_;
         const int g = slot0 + slot;
         const bool active = (slot < tpw_) && (g < total_slots);
@@ -390,6 +510,13 @@ struct Block {
 
     __device__ __forceinline__ void run() {
         const int last = nstages_ - 1;
+        if constexpr (!St::native) {  // 16-bit storage: complex-to-complex only
+            for (int s = 0; s <= last; ++s) {
+                const bool col = (s == 0) ? (in_col_ != 0) : (s == last) ? (out_col_ != 0) : false;
+                dispatch(s, s == 0, s == last, col);
+            }
+            return;
+        }
         if (mode_ == kModeC2R) real_pre();
         for (int s = 0; s <= last; ++s) {
             const bool src_g = (s == 0) && (mode_ != kModeC2R);

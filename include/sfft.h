@@ -53,6 +53,12 @@ extern "C" {
 /* precisions (the arithmetic type of the transform) */
 #define SFFT_F32 0 /* FFT<float>  */
 #define SFFT_F64 1 /* FFT<double> */
+/* 16-bit HBM storage, float arithmetic, complex-to-complex only (no
+ * counterpart in the reference): complex values are interleaved IEEE half /
+ * bfloat16 pairs (4 bytes); radix-16 stages run on the matrix cores with
+ * split hi+lo operands and float accumulation. */
+#define SFFT_F16 2
+#define SFFT_BF16 3
 
 /* directions */
 #define SFFT_FORWARD 0 /* fft()  */
@@ -69,6 +75,7 @@ typedef struct {
     int threads_per_transform, transforms_per_block;
     size_t workspace_bytes; /* device scratch held by the plan (0: none) */
     size_t table_bytes;     /* device twiddle tables held by the plan */
+    int mfma_stages;        /* radix-16 stages run as MFMA contractions (16-bit storage) */
 } sfft_plan_info;
 
 /* ---- library ---- */
