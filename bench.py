@@ -74,9 +74,10 @@ def copy_calibration(torch):
 class Launch:
     """One batched launch of the step: a plan, buffers, a direction."""
 
-    def __init__(self, name, plan, x, y, batch, direction, alg_bytes):
+    def __init__(self, name, plan, x, y, batch, direction, alg_bytes, flops):
         self.name, self.plan, self.x, self.y = name, plan, x, y
         self.batch, self.direction, self.alg_bytes = batch, direction, alg_bytes
+        self.flops = flops
         self.times_ms = []
 
     def run(self, stream_handle):
@@ -96,8 +97,9 @@ def build_workload(torch, fft_amd, L):
         z = torch.empty_like(x)
         bufs[n] = (x, y, z)
         ab = 16 * n * C2C_BATCH
-        launches.append(Launch(f"c2c_{n}_fwd", plan, x, y, C2C_BATCH, L.FORWARD, ab))
-        launches.append(Launch(f"c2c_{n}_inv", plan, y, z, C2C_BATCH, L.INVERSE, ab))
+        fl = 5 * n * math.log2(n) * C2C_BATCH
+        launches.append(Launch(f"c2c_{n}_fwd", plan, x, y, C2C_BATCH, L.FORWARD, ab, fl))
+        launches.append(Launch(f"c2c_{n}_inv", plan, y, z, C2C_BATCH, L.INVERSE, ab, fl))
     # RealFFT frames: tones from the seeded stream, noise generated in HBM
     n, frames = REAL_N, REAL_FRAMES
     fr = torch.from_numpy(synth.uniform01(n, frames * 8).reshape(frames, 8) * (n / 2)).cuda()
@@ -116,8 +118,9 @@ def build_workload(torch, fft_amd, L):
     bufs["real"] = (xr, yr, zr)
     plan = fft_amd.Plan(n, L.R2C, L.F32)
     ab = 8 * n * frames
-    launches.append(Launch(f"r2c_{n}_fwd", plan, xr, yr, frames, L.FORWARD, ab))
-    launches.append(Launch(f"c2r_{n}_inv", plan, yr, zr, frames, L.INVERSE, ab))
+    fl = 2.5 * n * math.log2(n) * frames
+    launches.append(Launch(f"r2c_{n}_fwd", plan, xr, yr, frames, L.FORWARD, ab, fl))
+    launches.append(Launch(f"c2r_{n}_inv", plan, yr, zr, frames, L.INVERSE, ab, fl))
     torch.cuda.synchronize()
     return launches, bufs
 
@@ -207,6 +210,55 @@ def uneven_batch(torch, fft_amd, count=20000, reps=5):
     del xin, out, b
     torch.cuda.empty_cache()
     return res
+
+
+def precision_sweep(torch, fft_amd, reps=5):
+    """Long 1D transforms N = 2^20 x 16 and 2^24 x 1, N(0,1) complex inputs
+    from a seed: FP64, FP32 and the 16-bit storage variants (IEEE half,
+    bfloat16; radix-16 MFMA stages).  Errors are against the FP64 path on the
+    same (storage-rounded) input values: rel L2 and max |error| / max |X|.
+    GB/s counts N complex values read + written in the variant's storage
+    (32 N bytes FP64, 16 N FP32, 8 N 16-bit)."""
+    from fft_amd import _lib as L
+    sh = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for n, batch, seed in ((1 << 20, 16, 20), (1 << 24, 1, 24)):
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        base = torch.randn(batch, n, 2, device="cuda", generator=g)
+        row = {}
+        for name, prec, dt, eb in (("f64", L.F64, torch.float64, 8), ("f32", L.F32, torch.float32, 4),
+                                   ("f16", L.F16, torch.float16, 2),
+                                   ("bf16", L.BF16, torch.bfloat16, 2)):
+            xs = base.to(dt).contiguous()
+            ys = torch.empty_like(xs)
+            plan = fft_amd.Plan(n, L.C2C, prec)
+            plan.execute_ptr(xs.data_ptr(), ys.data_ptr(), batch, L.FORWARD, sh)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                plan.execute_ptr(xs.data_ptr(), ys.data_ptr(), batch, L.FORWARD, sh)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            # FP64 reference on the same input values
+            xd = xs.double().contiguous()
+            yd = torch.empty_like(xd)
+            fft_amd.Plan(n, L.C2C, L.F64).execute_ptr(xd.data_ptr(), yd.data_ptr(), batch,
+                                                       L.FORWARD, sh)
+            d = ys.double() - yd
+            rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(yd))
+            mx = float(d.abs().max() / yd.abs().max())
+            info = plan.info()
+            row[name] = {"ms": round(ms, 4), "GB/s": round(2 * eb * 2 * n * batch / ms / 1e6, 1),
+                         "rel_l2_vs_fp64": float(f"{rel:.3e}"),
+                         "max_abs_err_rel_to_max": float(f"{mx:.3e}"),
+                         "passes": info["passes"], "mfma_stages": info["mfma_stages"]}
+            del xs, ys, xd, yd, d
+        out[f"n{n}_x{batch}"] = row
+        del base
+    torch.cuda.empty_cache()
+    return out
 
 
 def serving(rates=(2000, 10000, 50000), duration=0.25):
@@ -301,6 +353,7 @@ def main():
     launches, bufs = build_workload(torch, fft_amd, L)
     total_ms, wall = time_steps(torch, launches, args.steps, args.warmup)
     step_bytes = sum(ln.alg_bytes for ln in launches)
+    step_flops = sum(ln.flops for ln in launches)
     ms_per_step = total_ms / args.steps
     value = step_bytes / (ms_per_step * 1e-3) / 1e9
 
@@ -343,6 +396,8 @@ def main():
             "bytes_per_step": step_bytes,
         },
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "gflops": round(step_flops / (ms_per_step * 1e-3) / 1e9, 1),
+        "gflops_convention": "5 N log2 N per complex transform, 2.5 N log2 N per real one",
         "per_launch": per,
         "plain_per_call": None if plain_gbs is None else {
             "value": round(plain_gbs, 2), "unit": "GB/s",
@@ -357,6 +412,7 @@ def main():
     }
     if not args.no_extras:
         result["uneven_batch"] = uneven_batch(torch, fft_amd)
+        result["precision_sweep"] = precision_sweep(torch, fft_amd)
         result["serving"] = serving()
         sv = result["serving"]
         result["serving_p50_ms"] = sv["rate_10000"]["p50_ms"]

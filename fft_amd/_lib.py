@@ -63,6 +63,7 @@ class PlanInfo(ctypes.Structure):
         ("workspace_bytes", ctypes.c_size_t),
         ("table_bytes", ctypes.c_size_t),
         ("mfma_stages", ctypes.c_int),
+        ("chirp_length", ctypes.c_size_t),
     ]
 
 

@@ -77,7 +77,7 @@ class Plan:
             "threads_per_transform": inf.threads_per_transform,
             "transforms_per_block": inf.transforms_per_block,
             "workspace_bytes": inf.workspace_bytes, "table_bytes": inf.table_bytes,
-            "mfma_stages": inf.mfma_stages,
+            "mfma_stages": inf.mfma_stages, "chirp_length": inf.chirp_length,
         }
 
     def execute_ptr(self, in_ptr: int, out_ptr: int, batch: int, direction: int,

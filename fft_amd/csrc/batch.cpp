@@ -14,6 +14,10 @@ UnevenBatch::UnevenBatch(const sfft_batch_item *items, size_t count, int prec)
         if (it.direction != SFFT_FORWARD && it.direction != SFFT_INVERSE)
             throw Error(SFFT_ERR_INVALID_ARG, "batch item direction");
         plans_.push_back(get_plan(it.n, it.kind, prec));
+        if (plans_.back()->bluestein())
+            throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
+                        "size " + std::to_string(it.n) +
+                            ": lengths with prime factors > 13 run through sfft_execute only");
     }
     const size_t lds_cap = device_info().lds_bytes - kLdsReserve;
     if (prec == kF64)

@@ -76,9 +76,9 @@ size_t sfft_real_size_minimum(size_t n) { return sfft::real_size_minimum(n); }
 size_t sfft_real_size_maximum(size_t n) { return sfft::real_size_maximum(n); }
 
 int sfft_size_supported(size_t n, int kind) {
-    if (kind == SFFT_C2C) return sfft::supported_length(n) ? 1 : 0;
+    if (kind == SFFT_C2C) return sfft::plannable_length(n) ? 1 : 0;
     if (kind == SFFT_R2C || kind == SFFT_R2C_HALFSHIFT)
-        return (n >= 2 && n % 2 == 0 && sfft::supported_length(n / 2)) ? 1 : 0;
+        return (n >= 2 && n % 2 == 0 && sfft::plannable_length(n / 2)) ? 1 : 0;
     return 0;
 }
 
@@ -128,8 +128,11 @@ int sfft_plan_get_info(sfft_plan plan, sfft_plan_info *info) {
         info->n = p.n();
         info->kind = p.kind();
         info->precision = p.prec();
-        info->passes = (int)p.passes().size();
-        const sfft::PassDesc &pd = p.passes()[0].pd;
+        // a chirp-z plan reports the layout of its length-M inner FFT (run twice)
+        const sfft::Plan &q = p.bluestein() ? *p.bluestein_inner() : p;
+        info->passes = (int)q.passes().size() * (p.bluestein() ? 2 : 1);
+        info->chirp_length = p.bluestein_m();
+        const sfft::PassDesc &pd = q.passes()[0].pd;
         info->nradices = pd.nstages;
         for (int s = 0; s < pd.nstages && s < 16; ++s) info->radices[s] = pd.st[s].radix;
         info->threads_per_transform = pd.tpt;

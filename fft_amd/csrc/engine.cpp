@@ -135,6 +135,22 @@ static bool only_small_primes(size_t n) {
 
 bool supported_length(size_t n) { return n >= 1 && n <= (1ull << 28) && only_small_primes(n); }
 
+// Largest length a chirp-z plan takes (its convolution length stays <= 2^28).
+constexpr size_t kMaxBluestein = 1ull << 27;
+
+bool plannable_length(size_t n) { return supported_length(n) || (n >= 1 && n <= kMaxBluestein); }
+
+// Smallest convolution length >= 2N - 1 made of the factors 2, 3, 5.
+static size_t bluestein_length(size_t n) {
+    const size_t lo = 2 * n - 1;
+    size_t best = 0;
+    for (size_t p2 = 1; p2 < 4 * lo; p2 *= 2)
+        for (size_t p3 = p2; p3 < 4 * lo; p3 *= 3)
+            for (size_t p5 = p3; p5 < 4 * lo; p5 *= 5)
+                if (p5 >= lo && (!best || p5 < best)) best = p5;
+    return best;
+}
+
 static size_t elem_bytes(int prec) { return prec == kF64 ? 16 : 8; }
 static int max_block(int prec) { return prec >= kF16 ? kMaxBlock16 : 1024; }
 
@@ -267,9 +283,23 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     if (kind != kC2C && (n % 2))
         throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "real transforms need an even size");
     cn_ = kind == kC2C ? n : n / 2;
-    if (!supported_length(cn_))
-        throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
-                    "size " + std::to_string(n) + ": only factors 2, 3, 5, 7, 11, 13 are planned");
+    if (!supported_length(cn_)) {
+        if (prec >= kF16)
+            throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
+                        "16-bit storage: only lengths with factors 2, 3, 5, 7, 11, 13");
+        if (cn_ > kMaxBluestein)
+            throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "size " + std::to_string(n) + " too large");
+        bs_m_ = bluestein_length(cn_);
+        bs_inner_.reset(new Plan(bs_m_, kC2C, prec));
+        if (prec == kF64) {
+            build_tables<double>();
+            build_bluestein<double>();
+        } else {
+            build_tables<float>();
+            build_bluestein<float>();
+        }
+        return;
+    }
 
     PassPlan single;
     const bool force_two = h && h->split > 1;
@@ -368,7 +398,64 @@ int Plan::mfma_stages() const {
     return c;
 }
 
+template <typename T>
+void Plan::build_bluestein() {
+    const size_t eb = sizeof(C2<T>);
+    void *tmp = nullptr;
+    auto alloc = [&](void **p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess)
+            throw Error(SFFT_ERR_OUT_OF_MEMORY, "device allocation of plan tables failed");
+    };
+    alloc(&d_chirp_, cn_ * eb);
+    alloc(&d_bhat_, bs_m_ * eb);
+    table_bytes_ += (cn_ + bs_m_) * eb;
+    alloc(&tmp, bs_m_ * eb);
+    struct Free {
+        void *p;
+        ~Free() { (void)hipFree(p); }
+    } free_tmp{tmp};
+    check_hip(bs_gen_chirp<T>((C2<T> *)d_chirp_, (C2<T> *)tmp, cn_, bs_m_, nullptr),
+              "chirp generation");
+    bs_inner_->execute(tmp, bs_m_, d_bhat_, bs_m_, 1, 0, nullptr);
+    check_hip(bs_scale<T>((C2<T> *)d_bhat_, bs_m_, 1.0 / (double)bs_m_, nullptr), "chirp scale");
+    check_hip(hipDeviceSynchronize(), "chirp tables");
+}
+
+template <typename T>
+void Plan::run_bluestein(const void *in, int64_t din, void *out, int64_t dout, size_t batch,
+                         int inverse, hipStream_t stream) const {
+    const size_t row = bs_m_ * sizeof(C2<T>);
+    if (batch > (0x7fffffffull / bs_m_)) throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
+    void *ws = nullptr;
+    if (hipMallocAsync(&ws, 2 * row * batch, stream) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(SFFT_ERR_OUT_OF_MEMORY, "chirp-z workspace allocation failed");
+    }
+    C2<T> *a = (C2<T> *)ws, *c = (C2<T> *)((char *)ws + row * batch);
+    const bool real = kind_ != kC2C;
+    const int mode = !real ? kModeC2C : (inverse ? kModeC2RLoad : kModeR2CLoad);
+    const int hs = kind_ == kR2CHalf;
+    check_hip(bs_pre<T>((const C2<T> *)in, din, a, bs_m_, cn_, batch, (const C2<T> *)d_chirp_,
+                        mode, !real && inverse, hs, (const C2<T> *)d_real_tw_,
+                        (const C2<T> *)d_rot_, stream),
+              "chirp-z pre-multiply");
+    bs_inner_->execute(a, bs_m_, c, bs_m_, batch, 0, stream);
+    check_hip(bs_mul<T>(c, (const C2<T> *)d_bhat_, bs_m_, batch, stream), "chirp-z multiply");
+    bs_inner_->execute(c, bs_m_, a, bs_m_, batch, 1, stream);
+    check_hip(bs_post<T>(a, bs_m_, (C2<T> *)out, dout, cn_, batch, (const C2<T> *)d_chirp_,
+                         inverse, (real && inverse && hs) ? (const C2<T> *)d_rot_ : nullptr,
+                         stream),
+              "chirp-z post-multiply");
+    (void)hipFreeAsync(ws, stream);
+    if (real && !inverse)
+        check_hip(launch_real_post<T>((C2<T> *)out, dout, (int)cn_, batch,
+                                      (const C2<T> *)d_real_tw_, hs, stream),
+                  "real post-process launch");
+}
+
 Plan::~Plan() {
+    if (d_chirp_) (void)hipFree(d_chirp_);
+    if (d_bhat_) (void)hipFree(d_bhat_);
     if (d_tw_) (void)hipFree(d_tw_);
     if (d_real_tw_) (void)hipFree(d_real_tw_);
     if (d_rot_) (void)hipFree(d_rot_);
@@ -498,8 +585,7 @@ size_t Plan::slot_lds_bytes(size_t p) const {
 template <typename T>
 void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
                int inverse, hipStream_t stream) const {
-    const PassPlan &p0 = passes_[0];
-    if (p0.fixed && in_dist == default_in_dist(inverse) && out_dist == default_out_dist(inverse)) {
+    if (!passes_.empty() && passes_[0].fixed && in_dist == default_in_dist(inverse) && out_dist == default_out_dist(inverse)) {
         run_fixed<T>(in, out, batch, inverse, stream);
         return;
     }
@@ -513,6 +599,10 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
             if (in_dist % 2) throw Error(SFFT_ERR_INVALID_ARG, "real input distance must be even");
             din = (int64_t)(in_dist / 2);
         }
+    }
+    if (bs_inner_) {
+        run_bluestein<T>(in, din, out, dout, batch, inverse, stream);
+        return;
     }
     const size_t np = passes_.size();
     for (size_t p = 0; p < np; ++p) {

@@ -84,6 +84,11 @@ class Plan {
     const std::vector<PassPlan> &passes() const { return passes_; }
     size_t table_bytes() const { return table_bytes_; }
     int mfma_stages() const;
+    // Chirp-z plan: the length has a prime factor the kernels do not plan;
+    // the transform runs as a convolution through FFTs of length bluestein_m().
+    bool bluestein() const { return bs_inner_ != nullptr; }
+    size_t bluestein_m() const { return bs_m_; }
+    const Plan *bluestein_inner() const { return bs_inner_.get(); }
 
     // in/out: device pointers; dists in elements of the in/out type
     // (complex for C2C, real/complex for the real kinds).
@@ -111,6 +116,11 @@ class Plan {
              hipStream_t stream) const;
     template <typename T>
     void run_fixed(const void *in, void *out, size_t batch, int inverse, hipStream_t stream) const;
+    template <typename T>
+    void build_bluestein();
+    template <typename T>
+    void run_bluestein(const void *in, int64_t din, void *out, int64_t dout, size_t batch,
+                       int inverse, hipStream_t stream) const;
 
     size_t n_;
     int kind_, prec_;
@@ -125,6 +135,10 @@ class Plan {
     C2<double> *d_ptw_coarse_ = nullptr;
     int ptw_shift_ = 0;
     size_t table_bytes_ = 0;
+    std::unique_ptr<Plan> bs_inner_;
+    size_t bs_m_ = 0;
+    void *d_chirp_ = nullptr;
+    void *d_bhat_ = nullptr;
 };
 
 // Cached plans keyed by (n, kind, precision).
@@ -133,6 +147,8 @@ std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec);
 // Radix choice for a single pass of length n (exposed for tests/tuning).
 bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix = 25);
 bool supported_length(size_t n);
+// Any length a plan exists for in F32 / F64: direct, or chirp-z (up to 2^27).
+bool plannable_length(size_t n);
 
 void check_hip(hipError_t e, const char *what);
 

@@ -48,6 +48,23 @@ hipError_t gen_real_twiddles(C2<T> *tmi, C2<T> *rot, size_t n, int half_shift,
 hipError_t gen_post_twiddles(C2<double> *fine, C2<double> *coarse, int shift, size_t N,
                              hipStream_t stream);
 
+// Bluestein (chirp-z) helpers, bluestein.hip.  w: chirp of length N;
+// b: convolution kernel of length M; a / c: per-transform rows of length M.
+template <typename T>
+hipError_t bs_gen_chirp(C2<T> *w, C2<T> *b, size_t n_len, size_t m_len, hipStream_t stream);
+template <typename T>
+hipError_t bs_scale(C2<T> *z, size_t m_len, double s, hipStream_t stream);
+template <typename T>
+hipError_t bs_pre(const C2<T> *in, int64_t in_dist, C2<T> *a, size_t m_len, size_t n_len,
+                  size_t batch, const C2<T> *w, int mode, int conj_in, int half_shift,
+                  const C2<T> *tmi, const C2<T> *rot, hipStream_t stream);
+template <typename T>
+hipError_t bs_mul(C2<T> *c, const C2<T> *bhat, size_t m_len, size_t batch, hipStream_t stream);
+template <typename T>
+hipError_t bs_post(const C2<T> *c, size_t m_len, C2<T> *out, int64_t out_dist, size_t n_len,
+                   size_t batch, const C2<T> *w, int conj_out, const C2<T> *rotc,
+                   hipStream_t stream);
+
 // Counter-based splitmix64 inputs (same stream as fft_amd/synth.py).
 hipError_t fill_synthetic(void *dst, size_t count, int f64, int dist, uint64_t seed,
                           hipStream_t stream);

@@ -76,6 +76,8 @@ typedef struct {
     size_t workspace_bytes; /* device scratch held by the plan (0: none) */
     size_t table_bytes;     /* device twiddle tables held by the plan */
     int mfma_stages;        /* radix-16 stages run as MFMA contractions (16-bit storage) */
+    size_t chirp_length;    /* lengths with prime factors > 13: the plan is a chirp-z
+                               convolution through FFTs of this length (0: direct) */
 } sfft_plan_info;
 
 /* ---- library ---- */
@@ -97,7 +99,10 @@ size_t sfft_size_maximum(size_t n);
  * including the reference's quirks (sizeMaximum may exceed n). */
 size_t sfft_real_size_minimum(size_t n);
 size_t sfft_real_size_maximum(size_t n);
-/* 1 if the engine plans this length for the kind, else 0. */
+/* 1 if the engine plans this length for the kind (F32 / F64), else 0.  Every
+ * length up to 2^27 is planned, as the reference accepts every length
+ * (signalsmith-fft.h:187-215 handles any factor): lengths whose factors are
+ * all <= 13 run directly, the others as a chirp-z convolution. */
 int sfft_size_supported(size_t n, int kind);
 
 /* ---- plans ---- */

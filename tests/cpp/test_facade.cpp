@@ -50,9 +50,9 @@ static bool close_enough(const std::vector<cd> &a, const std::vector<cd> &b) {
     return std::sqrt(err / energy) < 1e-6;
 }
 
-static std::vector<int> test_sizes() {  // tests/00-fft.cpp:8-16 (the planned ones)
+static std::vector<int> test_sizes() {  // tests/00-fft.cpp:8-16
     return {1, 2, 4, 8, 16, 32, 64, 128, 256, 3, 6, 9, 12, 18, 24, 5, 10, 15, 20, 25,
-            7, 14, 21, 28, 49, 11, 13, 22};
+            7, 14, 21, 28, 49, 11, 13, 17, 19, 22, 23};
 }
 
 static void individual_bins() {
@@ -174,13 +174,14 @@ static void return_values() {  // setSize quirks (signalsmith-fft.h:356-372, :41
     CHECK(signalsmith::RealFFT<float>::sizeMaximum(10) == 12, "RealFFT sizeMaximum quirk");
     CHECK(signalsmith::FFT<double>(100, 1).size() == 128, "fastDirection +1");
     CHECK(signalsmith::FFT<double>(100, -1).size() == 96, "fastDirection -1");
+    CHECK(signalsmith::FFT<double>(17).size() == 17, "prime length plans (chirp-z)");
     bool threw = false;
     try {
-        signalsmith::FFT<double> bad(17);
+        signalsmith::FFT<double> bad((size_t(1) << 28) + 1);
     } catch (const signalsmith::FFTError &e) {
         threw = e.code == SFFT_ERR_UNSUPPORTED_SIZE;
     }
-    CHECK(threw, "unplanned size reports SFFT_ERR_UNSUPPORTED_SIZE");
+    CHECK(threw, "oversize length reports SFFT_ERR_UNSUPPORTED_SIZE");
 }
 
 static void float_device_batch() {  // device pointers, a batch, FFT<float>

@@ -53,7 +53,9 @@ def test_size_supported_flags():
     assert lib.sfft_size_supported(0, L.C2C) == 0
     assert lib.sfft_size_supported(2048, L.R2C) == 1
     assert lib.sfft_size_supported(2049, L.R2C) == 0
-    assert lib.sfft_size_supported(17, L.C2C) == 0
+    assert lib.sfft_size_supported(17, L.C2C) == 1  # chirp-z plan
+    assert lib.sfft_size_supported(34, L.R2C) == 1
+    assert lib.sfft_size_supported((1 << 28) + 1, L.C2C) == 0
 
 
 def test_last_error_is_empty_string_initially():

@@ -23,7 +23,7 @@ TOL32 = 1e-6   # rel L2 vs the FP64 oracle, FP32 path
 TOL64 = 1e-13  # rel L2 vs reference FFT<double>, FP64 path
 
 TEST_SIZES = [1, 2, 4, 8, 16, 32, 64, 128, 256, 3, 6, 9, 12, 18, 24, 5, 10, 15, 20, 25,
-              7, 14, 21, 28, 49, 11, 13, 22]  # tests/00-fft.cpp:8-16 (17, 19, 23 below)
+              7, 14, 21, 28, 49, 11, 13, 17, 19, 22, 23]  # tests/00-fft.cpp:8-16
 
 
 def _dev(x):
@@ -119,11 +119,49 @@ def test_linearity_and_inverse(n):
     assert po.close_enough_metric(back1, n * h1) < 1e-6
 
 
-@pytest.mark.parametrize("n", [17, 19, 23])
-def test_large_prime_sizes_rejected_with_status(n):
-    assert L.lib().sfft_size_supported(n, L.C2C) == 0
+@pytest.mark.parametrize("n", [17, 19, 23, 29, 97, 289, 1009, 2 * 3 * 17 * 19, 65537, 100003])
+def test_chirp_z_sizes(n):
+    # prime factors > 13 (tests/00-fft.cpp:14 tests 17, 19, 23): chirp-z plans,
+    # checked against the oracle's generic-radix restatement in both precisions
+    info = fft_amd.Plan(n, L.C2C, L.F64).info()
+    assert info["chirp_length"] >= 2 * n - 1
+    # The reference's generic radix computes phases 2 pi i f / len in double
+    # (signalsmith-fft.h:204-205), which costs it ~1e-13 at N ~ 1000 and more
+    # beyond: the exact DFT (long-double brute force, or numpy's FFT above
+    # 4096) pins the chirp-z values, the oracle stays within its own error.
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n)))
+    for inv in (False, True):
+        sign = 1 if inv else -1
+        if n <= 4096:
+            exact = np.stack([po.dft_ld(r, sign) for r in x])
+        else:
+            exact = np.fft.ifft(x) * n if inv else np.fft.fft(x)
+        y64 = _run(n, x, inverse=inv, dtype=torch.complex128)
+        assert po.rel_l2(y64, exact) <= TOL64, (n, inv, po.rel_l2(y64, exact))
+        if n <= 4096:  # the oracle's prime step is O(N^2)
+            assert po.rel_l2(y64, po.fft(x, inverse=inv)) <= 1e-9, (n, inv)
+        y32 = _run(n, x.astype(np.complex64), inverse=inv)
+        assert po.rel_l2(y32, exact) <= TOL32, (n, inv, po.rel_l2(y32, exact))
+
+
+def test_chirp_z_round_trip_and_bins():
+    # single bins and ifft(fft(x)) = N x (tests/00-fft.cpp:19-64, :103-130) at N = 23
+    n = 23
+    for b in range(n):
+        h = np.exp(2j * np.pi * b * np.arange(n) / n)
+        y = _run(n, h, dtype=torch.complex128)
+        want = np.zeros(n, complex)
+        want[b] = n
+        assert po.close_enough_metric(y, want) < 1e-6, b
+    x = np.exp(1j * np.arange(n) ** 2)
+    back = _run(n, _run(n, x, dtype=torch.complex128), inverse=True, dtype=torch.complex128)
+    assert po.rel_l2(back, n * x) < 1e-13
+
+
+def test_oversize_rejected_with_status():
     with pytest.raises(L.SfftError) as e:
-        fft_amd.FFT(n)
+        fft_amd.Plan((1 << 28) + 1, L.C2C, L.F32)
     assert e.value.code == L.SFFT_ERR_UNSUPPORTED_SIZE
 
 
