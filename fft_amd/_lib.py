@@ -32,6 +32,7 @@ EXPORTED = [
     "sfft_real_size_maximum", "sfft_size_supported",
     "sfft_plan_create", "sfft_plan_destroy", "sfft_plan_get_info",
     "sfft_plan_create_hinted", "sfft_plan_variants",
+    "sfft_autotune", "sfft_tune_save", "sfft_tune_load", "sfft_tune_clear",
     "sfft_execute", "sfft_execute_strided",
     "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_execute",
     "sfft_batch_execute_per_size",
@@ -74,6 +75,18 @@ class PlanHints(ctypes.Structure):
         ("radices", ctypes.c_int * 16),
         ("transforms_per_block", ctypes.c_int),
         ("split", ctypes.c_size_t),
+        ("elements_per_thread", ctypes.c_int),
+        ("mfma", ctypes.c_int),
+    ]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("hints", PlanHints),
+        ("ms", ctypes.c_double),
+        ("gbs", ctypes.c_double),
+        ("batch", ctypes.c_size_t),
+        ("candidates", ctypes.c_int),
     ]
 
 
@@ -158,6 +171,9 @@ def lib() -> ctypes.CDLL:
         L.sfft_plan_create_hinted.argtypes = [ctypes.POINTER(vp), sz, ip, ip,
                                               ctypes.POINTER(PlanHints)]
         L.sfft_plan_variants.argtypes = [sz, ip, ip]
+        L.sfft_autotune.argtypes = [sz, ip, ip, sz, ctypes.POINTER(TuneResult)]
+        L.sfft_tune_save.argtypes = [ctypes.c_char_p]
+        L.sfft_tune_load.argtypes = [ctypes.c_char_p, ip, ctypes.POINTER(sz)]
         L.sfft_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
         L.sfft_execute.argtypes = [vp, vp, vp, sz, ip, vp]
         L.sfft_execute_strided.argtypes = [vp, vp, sz, vp, sz, sz, ip, vp]

@@ -38,11 +38,14 @@ class Plan:
     """An engine plan: (n, kind, precision) -> device twiddles + launch layout."""
 
     def __init__(self, n: int, kind: int = L.C2C, precision: int = L.F32, variant=None,
-                 radices=None, transforms_per_block: int = 0, split: int = 0):
-        """variant / radices / transforms_per_block / split override the
-        planner (sfft_plan_create_hinted); without them the cached plan is used."""
+                 radices=None, transforms_per_block: int = 0, split: int = 0,
+                 elements_per_thread: int = 0, mfma: int = -1):
+        """variant / radices / transforms_per_block / split / elements_per_thread /
+        mfma override the planner (sfft_plan_create_hinted); without them the
+        cached (possibly autotuned) plan is used."""
         h = ctypes.c_void_p()
-        if variant is None and radices is None and not transforms_per_block and not split:
+        if (variant is None and radices is None and not transforms_per_block and not split
+                and not elements_per_thread and mfma == -1):
             L.check(L.lib().sfft_plan_create(ctypes.byref(h), n, kind, precision))
         else:
             hints = L.PlanHints()
@@ -53,6 +56,8 @@ class Plan:
                 hints.radices[i] = int(r)
             hints.transforms_per_block = int(transforms_per_block)
             hints.split = int(split)
+            hints.elements_per_thread = int(elements_per_thread)
+            hints.mfma = int(mfma)
             L.check(L.lib().sfft_plan_create_hinted(ctypes.byref(h), n, kind, precision,
                                                     ctypes.byref(hints)))
         self._h = h
@@ -347,3 +352,36 @@ def fill(t: torch.Tensor, seed: int, dist: str = "uniform", stream=None) -> torc
                               0 if dist == "uniform" else 1, seed,
                               ctypes.c_void_p(_stream_handle(stream))))
     return t
+
+
+def _hints_dict(h) -> dict:
+    return {"variant": h.variant, "radices": [h.radices[i] for i in range(h.nradices)],
+            "transforms_per_block": h.transforms_per_block, "split": h.split,
+            "elements_per_thread": h.elements_per_thread, "mfma": h.mfma}
+
+
+def autotune(n: int, kind: int = L.C2C, precision: int = L.F32, batch: int = 0) -> dict:
+    """Time the planner's candidates for (n, kind, precision) on the GPU and
+    make the fastest the plan Plan(n, kind, precision) returns from now on
+    (sfft_autotune).  batch 0: about 256 MB per timed launch."""
+    r = L.TuneResult()
+    L.check(L.lib().sfft_autotune(n, kind, precision, batch, ctypes.byref(r)))
+    return {"hints": _hints_dict(r.hints), "ms": r.ms, "GB/s": r.gbs, "batch": r.batch,
+            "candidates": r.candidates}
+
+
+def save_tuning(path: str) -> None:
+    """Write the tuned choices as JSON (keyed by GPU architecture and HIP runtime)."""
+    L.check(L.lib().sfft_tune_save(str(path).encode()))
+
+
+def load_tuning(path: str, check_device: bool = True) -> int:
+    """Read tuned choices; returns how many were taken (0 if the file was
+    tuned on another architecture or runtime and check_device is set)."""
+    k = ctypes.c_size_t(0)
+    L.check(L.lib().sfft_tune_load(str(path).encode(), 1 if check_device else 0, ctypes.byref(k)))
+    return k.value
+
+
+def clear_tuning() -> None:
+    L.check(L.lib().sfft_tune_clear())

@@ -1,12 +1,14 @@
 // capi.cpp -- the C ABI (include/sfft.h) over the engine.
 #include <sfft.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
 
 #include "batch.h"
 #include "engine.h"
+#include "tune.h"
 
 namespace sfft {
 std::string &last_error() {
@@ -92,18 +94,75 @@ int sfft_plan_create(sfft_plan *plan, size_t n, int kind, int precision) {
     });
 }
 
+static sfft::PlanHints from_abi(const sfft_plan_hints &a) {
+    sfft::PlanHints h;
+    h.fixed_variant = a.variant;
+    for (int i = 0; i < a.nradices && i < 16; ++i) h.radices.push_back(a.radices[i]);
+    h.tpw = a.transforms_per_block;
+    h.split = a.split;
+    h.ept = a.elements_per_thread;
+    h.mfma = a.mfma;
+    return h;
+}
+
+static sfft_plan_hints to_abi(const sfft::PlanHints &h) {
+    sfft_plan_hints a;
+    std::memset(&a, 0, sizeof(a));
+    a.variant = h.fixed_variant;
+    a.nradices = (int)std::min<size_t>(h.radices.size(), 16);
+    for (int i = 0; i < a.nradices; ++i) a.radices[i] = h.radices[i];
+    a.transforms_per_block = h.tpw;
+    a.split = h.split;
+    a.elements_per_thread = h.ept;
+    a.mfma = h.mfma;
+    return a;
+}
+
+int sfft_autotune(size_t n, int kind, int precision, size_t batch, sfft_tune_result *result) {
+    return guarded([&] {
+        const sfft::TuneResult t = sfft::autotune(n, kind, precision, batch, 5);
+        if (result) {
+            result->hints = to_abi(t.hints);
+            result->ms = t.ms;
+            result->gbs = t.gbs;
+            result->batch = t.batch;
+            result->candidates = t.candidates;
+        }
+        return SFFT_OK;
+    });
+}
+
+int sfft_tune_save(const char *path) {
+    return guarded([&] {
+        if (!path) return fail(SFFT_ERR_INVALID_ARG, "NULL path");
+        sfft::tune_save(path);
+        return SFFT_OK;
+    });
+}
+
+int sfft_tune_load(const char *path, int check_device, size_t *loaded) {
+    return guarded([&] {
+        if (!path) return fail(SFFT_ERR_INVALID_ARG, "NULL path");
+        const size_t k = sfft::tune_load(path, check_device != 0);
+        if (loaded) *loaded = k;
+        return SFFT_OK;
+    });
+}
+
+int sfft_tune_clear(void) {
+    return guarded([&] {
+        sfft::tune_clear();
+        return SFFT_OK;
+    });
+}
+
 int sfft_plan_create_hinted(sfft_plan *plan, size_t n, int kind, int precision,
                             const sfft_plan_hints *hints) {
     return guarded([&] {
         if (!plan) return fail(SFFT_ERR_INVALID_ARG, "plan pointer is NULL");
         *plan = nullptr;
         sfft::PlanHints h;
-        if (hints) {
-            h.fixed_variant = hints->variant;
-            for (int i = 0; i < hints->nradices && i < 16; ++i) h.radices.push_back(hints->radices[i]);
-            h.tpw = hints->transforms_per_block;
-            h.split = hints->split;
-        }
+        if (hints) h = from_abi(*hints);
         auto p = std::make_shared<sfft::Plan>(n, kind, precision, &h);
         *plan = new sfft_plan_s{p};
         return SFFT_OK;

@@ -1,5 +1,6 @@
 // engine.cpp -- planner, device tables, plan cache and pass execution.
 #include "engine.h"
+#include "tune.h"
 
 #include <sfft.h>
 
@@ -7,6 +8,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <cstdlib>
 #include <tuple>
 #include <type_traits>
 
@@ -381,7 +383,7 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         passes_.push_back(p1);
         passes_.push_back(p2);
     }
-    if (prec >= kF16) {
+    if (prec >= kF16 && !(h && h->mfma == 0)) {
         for (auto &pp : passes_) enable_mfma(pp);
     }
     if (prec == kF64)
@@ -687,16 +689,134 @@ void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, s
         run<float>(in, in_dist, out, out_dist, batch, inverse, stream);
 }
 
+static std::mutex g_cache_mu;
+static std::map<std::tuple<size_t, int, int>, std::shared_ptr<Plan>> g_cache;
+
 std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec) {
-    static std::mutex mu;
-    static std::map<std::tuple<size_t, int, int>, std::shared_ptr<Plan>> cache;
-    std::lock_guard<std::mutex> lock(mu);
+    // tuned choices named by SFFT_TUNE_FILE, read once (before the cache lock:
+    // loading drops cached plans)
+    static std::once_flag tune_file_once;
+    std::call_once(tune_file_once, [] {
+        const char *f = std::getenv("SFFT_TUNE_FILE");
+        if (!f || !*f) return;
+        try {
+            (void)tune_load(f, true);
+        } catch (const Error &) {
+            // an unreadable tuning file leaves the planner's defaults
+        }
+    });
+    std::lock_guard<std::mutex> lock(g_cache_mu);
     auto key = std::make_tuple(n, kind, prec);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    auto p = std::make_shared<Plan>(n, kind, prec);
-    cache[key] = p;
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) return it->second;
+    PlanHints h;
+    std::shared_ptr<Plan> p;
+    if (tuned_hints(n, kind, prec, h)) {
+        try {
+            p = std::make_shared<Plan>(n, kind, prec, &h);
+        } catch (const Error &) {
+            p = nullptr;  // a stale tuned entry never blocks the default plan
+        }
+    }
+    if (!p) p = std::make_shared<Plan>(n, kind, prec);
+    g_cache[key] = p;
     return p;
+}
+
+void drop_cached_plan(size_t n, int kind, int prec) {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    g_cache.erase(std::make_tuple(n, kind, prec));
+}
+
+// Every radix multiset (non-increasing order) for length n within the
+// thread budget, with its cost.
+static void collect_radices(int rem, int max_r, std::vector<int> &cur, int n, int ept,
+                            int max_threads, std::vector<std::pair<double, std::vector<int>>> &out) {
+    if (rem == 1) {
+        const int tpt = threads_for(n, cur, ept);
+        if (tpt <= max_threads) out.push_back({radix_cost(n, cur, ept, tpt), cur});
+        return;
+    }
+    if ((int)cur.size() >= kMaxStages || out.size() > 4096) return;
+    for (int r : kRadixSet) {
+        if (r > max_r || r < 2 || rem % r) continue;
+        cur.push_back(r);
+        collect_radices(rem / r, r, cur, n, ept, max_threads, out);
+        cur.pop_back();
+    }
+}
+
+std::vector<PlanHints> plan_candidates(size_t n, int kind, int prec) {
+    std::vector<PlanHints> out;
+    out.push_back(PlanHints{});  // the planner's choice
+    const size_t cn = kind == kC2C ? n : n / 2;
+    if (!supported_length(cn) || cn < 2) return out;
+    const int real = kind != kC2C;
+    const int nv = fixed_variant_count((int)cn, prec, real);
+    for (int v = 0; v < nv; ++v) {
+        PlanHints h;
+        h.fixed_variant = v;
+        out.push_back(h);
+    }
+    if (prec >= kF16) {
+        PlanHints h;
+        h.mfma = 0;
+        out.push_back(h);
+    }
+    const int maxr = prec == kF64 ? 16 : 25;
+    if (cn <= (1u << 20)) {
+        for (int ept : {kEptSmall, kEptLarge}) {
+            if (prec == kF64 && ept > kEptSmall) continue;
+            std::vector<std::pair<double, std::vector<int>>> all;
+            std::vector<int> cur;
+            collect_radices((int)cn, maxr, cur, (int)cn, ept, max_block(prec), all);
+            std::stable_sort(all.begin(), all.end(),
+                             [](const auto &a, const auto &b) { return a.first < b.first; });
+            const size_t keep = std::min<size_t>(all.size(), 3);
+            for (size_t i = 0; i < keep; ++i) {
+                std::vector<int> desc = all[i].second, asc(desc.rbegin(), desc.rend());
+                for (const auto &order : {desc, asc}) {
+                    PlanHints h;
+                    h.radices = order;
+                    h.ept = ept;
+                    out.push_back(h);
+                    if (order == asc && asc == desc) break;
+                }
+                if (i == 0) {  // tile widths around the default for the best multiset
+                    const int tpt = threads_for((int)cn, desc, ept);
+                    const int def = std::max(1, 256 / tpt);
+                    for (int tpw : {def / 2, def * 2}) {
+                        if (tpw < 1 || tpw * tpt > max_block(prec) || tpw == def) continue;
+                        PlanHints h;
+                        h.radices = desc;
+                        h.ept = ept;
+                        h.tpw = tpw;
+                        out.push_back(h);
+                    }
+                }
+            }
+        }
+    }
+    // two-pass splits (and one pass where both exist)
+    std::vector<std::pair<double, size_t>> splits;
+    for (size_t n1 = 32; n1 * 16 <= cn; ++n1) {
+        if (cn % n1) continue;
+        const size_t n2 = cn / n1;
+        PassPlan p1, p2;
+        if (!layout_columns((int)n1, prec, (int)std::min<size_t>(n2, 16), nullptr, p1)) continue;
+        if (!layout_columns((int)n2, prec, (int)std::min<size_t>(n1, 16), nullptr, p2)) continue;
+        if (p1.pd.nstages < 2) continue;
+        const double score = -std::log2((double)std::min(p1.pd.tpw, p2.pd.tpw)) * 4.0 +
+                             std::fabs(std::log2((double)n1) - std::log2((double)n2));
+        splits.push_back({score, n1});
+    }
+    std::stable_sort(splits.begin(), splits.end());
+    for (size_t i = 0; i < std::min<size_t>(splits.size(), 3); ++i) {
+        PlanHints h;
+        h.split = splits[i].second;
+        out.push_back(h);
+    }
+    return out;
 }
 
 }  // namespace sfft

@@ -68,6 +68,7 @@ struct PlanHints {
     int ept = 0;               // elements-per-thread instance (0: planner's)
     size_t split = 0;          // N1 of a two-pass plan (0: planner's; 1: force single pass)
     int fixed_variant = -1;    // -1 default, -2 no fixed kernel, k >= 0 that variant
+    int mfma = -1;             // 16-bit storage: -1 default (on), 0 scalar radix-16 butterflies
 };
 
 class Plan {
@@ -140,6 +141,16 @@ class Plan {
     void *d_chirp_ = nullptr;
     void *d_bhat_ = nullptr;
 };
+
+// Candidate planner overrides for the autotuner: the planner's default
+// first, then compiled variants, radix multisets and orders, tile widths,
+// elements per thread, two-pass splits (and MFMA off for 16-bit storage).
+std::vector<PlanHints> plan_candidates(size_t n, int kind, int prec);
+
+// Tuned overrides (tune.cpp): get_plan builds with them when present.
+bool tuned_hints(size_t n, int kind, int prec, PlanHints &out);
+// Drops the cached plan of (n, kind, prec) so the next get_plan rebuilds it.
+void drop_cached_plan(size_t n, int kind, int prec);
 
 // Cached plans keyed by (n, kind, precision).
 std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec);

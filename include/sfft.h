@@ -124,6 +124,8 @@ typedef struct {
     int radices[16];
     int transforms_per_block; /* 0: planner's choice */
     size_t split;         /* two-pass N1 (0: planner's choice, 1: force one pass) */
+    int elements_per_thread;  /* 16 or 32 (0: planner's choice) */
+    int mfma;             /* 16-bit storage: -1 default (radix-16 stages on MFMA), 0 scalar */
 } sfft_plan_hints;
 
 /* A plan built with hints (not cached, not used by sfft_plan_create). */
@@ -131,6 +133,29 @@ int sfft_plan_create_hinted(sfft_plan *plan, size_t n, int kind, int precision,
                             const sfft_plan_hints *hints);
 /* Number of compiled fixed-plan schedules for (n, kind, precision). */
 int sfft_plan_variants(size_t n, int kind, int precision);
+
+/* ---- autotuner (no counterpart in the reference) ----
+ * sfft_autotune times the planner's candidates for (n, kind, precision) --
+ * compiled schedule variants, radix multisets and their order, transforms
+ * per block, elements per thread, one pass against two-pass splits, MFMA on
+ * or off for 16-bit storage -- over `batch` transforms of synthetic data in
+ * HBM (0: about 256 MB per launch), forward and inverse, and makes the
+ * fastest the plan that sfft_plan_create returns for that key from then on.
+ * Tuned choices are saved / loaded as JSON keyed by the GPU architecture and
+ * the HIP runtime version; a file from another device or runtime is ignored
+ * when check_device is set.  The environment variable SFFT_TUNE_FILE names a
+ * file loaded (checked) at the first plan creation. */
+typedef struct {
+    sfft_plan_hints hints;  /* the winner */
+    double ms;              /* per launch, mean of forward and inverse */
+    double gbs;             /* algorithmic bytes (input + output) per second / 1e9 */
+    size_t batch;           /* transforms per timed launch */
+    int candidates;         /* candidates timed */
+} sfft_tune_result;
+int sfft_autotune(size_t n, int kind, int precision, size_t batch, sfft_tune_result *result);
+int sfft_tune_save(const char *path);
+int sfft_tune_load(const char *path, int check_device, size_t *loaded);
+int sfft_tune_clear(void);
 
 /* FFT<V>::fft / ifft and RealFFT<V,flags>::fft / ifft
  * (signalsmith-fft.h:374-386, :446-502) over a batch of device buffers.
