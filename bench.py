@@ -183,17 +183,27 @@ def plain_per_call(torch, fft_amd, L, bufs, sample):
 
 def uneven_batch(torch, fft_amd, count=20000, reps=5):
     """The uneven batch: 20000 transforms, fast sizes log-uniform in [64, 65536],
-    direction 50/50, U[-1,1) inputs; one persistent graph-captured launch vs the
-    plain per-size loop (one batched launch per item)."""
+    direction 50/50, U[-1,1) inputs; one persistent graph-captured launch with
+    the planner's plans, then with autotuned plans (every distinct size tuned
+    first), vs the plain per-size loop (one batched launch per item)."""
+    from fft_amd import _lib as L
     from fft_amd import workloads
     sizes, dirs = workloads.uneven_batch(seed=3, count=count)
+    fft_amd.clear_tuning()
     b = fft_amd.UnevenBatch([int(x) for x in sizes], [int(d) for d in dirs])
     xin = torch.empty(b.in_bytes, dtype=torch.uint8, device="cuda")
     fft_amd.fill(xin.view(torch.float32), seed=3)
     out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
     alg = b.in_bytes + b.out_bytes
     res = {}
-    for name, fn in (("persistent", b.execute), ("per_size_loop", b.execute_per_size)):
+    t0 = time.perf_counter()
+    uniq = sorted(set(int(x) for x in sizes))
+    for n in uniq:
+        fft_amd.autotune(n, L.C2C, L.F32, 0 if n > 4096 else 2048)
+    tune_s = time.perf_counter() - t0
+    bt = fft_amd.UnevenBatch([int(x) for x in sizes], [int(d) for d in dirs])
+    for name, fn in (("persistent", b.execute), ("persistent_tuned", bt.execute),
+                     ("per_size_loop", b.execute_per_size)):
         fn(xin, out)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -205,9 +215,12 @@ def uneven_batch(torch, fft_amd, count=20000, reps=5):
         ms = e0.elapsed_time(e1) / reps
         res[name] = {"ms": round(ms, 3), "GB/s": round(alg / ms / 1e6, 1)}
     res["speedup"] = round(res["per_size_loop"]["ms"] / res["persistent"]["ms"], 2)
+    res["speedup_tuned"] = round(res["per_size_loop"]["ms"] / res["persistent_tuned"]["ms"], 2)
+    res["autotune"] = {"sizes": len(uniq), "seconds": round(tune_s, 2)}
+    fft_amd.clear_tuning()
     res["transforms"] = count
     res["alg_bytes"] = alg
-    del xin, out, b
+    del xin, out, b, bt
     torch.cuda.empty_cache()
     return res
 
