@@ -199,7 +199,7 @@ def uneven_batch(torch, fft_amd, count=20000, reps=5):
     t0 = time.perf_counter()
     uniq = sorted(set(int(x) for x in sizes))
     for n in uniq:
-        fft_amd.autotune(n, L.C2C, L.F32, 0 if n > 4096 else 2048)
+        fft_amd.autotune(n, L.C2C, L.F32, max(1, (128 << 20) // (16 * n)))
     tune_s = time.perf_counter() - t0
     bt = fft_amd.UnevenBatch([int(x) for x in sizes], [int(d) for d in dirs])
     for name, fn in (("persistent", b.execute), ("persistent_tuned", bt.execute),

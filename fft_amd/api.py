@@ -363,7 +363,7 @@ def _hints_dict(h) -> dict:
 def autotune(n: int, kind: int = L.C2C, precision: int = L.F32, batch: int = 0) -> dict:
     """Time the planner's candidates for (n, kind, precision) on the GPU and
     make the fastest the plan Plan(n, kind, precision) returns from now on
-    (sfft_autotune).  batch 0: about 256 MB per timed launch."""
+    (sfft_autotune).  batch 0: about 1 GB per timed launch."""
     r = L.TuneResult()
     L.check(L.lib().sfft_autotune(n, kind, precision, batch, ctypes.byref(r)))
     return {"hints": _hints_dict(r.hints), "ms": r.ms, "GB/s": r.gbs, "batch": r.batch,

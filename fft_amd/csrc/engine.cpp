@@ -341,14 +341,18 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         // N1 column FFTs of length N2 in place (stride N1).
         size_t best_n1 = 0;
         double best_score = 1e30;
+        // column tile width: a hint's transforms_per_block caps it.  8 columns
+        // (64 B rows in F32) measured faster than 16 on most lengths: smaller
+        // tiles give more resident blocks per CU.
+        const size_t max_cols = (h && h->tpw) ? (size_t)h->tpw : 8;
         for (size_t n1 = 32; n1 * 16 <= cn_; ++n1) {
             if (cn_ % n1) continue;
             if (h && h->split > 1 && n1 != h->split) continue;
             const size_t n2 = cn_ / n1;
             PassPlan p1, p2;
-            if (!layout_columns((int)n1, prec, (int)std::min<size_t>(n2, 16), nullptr, p1))
+            if (!layout_columns((int)n1, prec, (int)std::min(n2, max_cols), nullptr, p1))
                 continue;
-            if (!layout_columns((int)n2, prec, (int)std::min<size_t>(n1, 16), nullptr, p2))
+            if (!layout_columns((int)n2, prec, (int)std::min(n1, max_cols), nullptr, p2))
                 continue;
             if (p1.pd.nstages < 2) continue;
             // prefer wide column tiles (coalescing), then balance
@@ -362,8 +366,8 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         if (!best_n1) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "no two-pass split");
         const size_t n1 = best_n1, n2 = cn_ / n1;
         PassPlan p1, p2;
-        layout_columns((int)n1, prec, (int)std::min<size_t>(n2, 16), nullptr, p1);
-        layout_columns((int)n2, prec, (int)std::min<size_t>(n1, 16), nullptr, p2);
+        layout_columns((int)n1, prec, (int)std::min(n2, max_cols), nullptr, p1);
+        layout_columns((int)n2, prec, (int)std::min(n1, max_cols), nullptr, p2);
         p1.slots_per_transform = (int)n2;
         p1.in_slot_stride = 1;
         p1.in_elem_stride = (int64_t)n2;
@@ -812,9 +816,12 @@ std::vector<PlanHints> plan_candidates(size_t n, int kind, int prec) {
     }
     std::stable_sort(splits.begin(), splits.end());
     for (size_t i = 0; i < std::min<size_t>(splits.size(), 3); ++i) {
-        PlanHints h;
-        h.split = splits[i].second;
-        out.push_back(h);
+        for (int cols : {0, 16, 4}) {  // column tile width (0: the planner's, 8)
+            PlanHints h;
+            h.split = splits[i].second;
+            h.tpw = cols;
+            out.push_back(h);
+        }
     }
     return out;
 }

@@ -17,7 +17,7 @@ struct TuneResult {
 };
 
 // Times every candidate of plan_candidates() over `batch` transforms (0:
-// about 256 MB per launch), forward and inverse, `reps` launches each; the
+// about 1 GB per launch), forward and inverse, `reps` launches each; the
 // fastest becomes the plan get_plan() builds for (n, kind, prec).
 TuneResult autotune(size_t n, int kind, int prec, size_t batch, int reps);
 void tune_clear();

@@ -139,7 +139,7 @@ int sfft_plan_variants(size_t n, int kind, int precision);
  * compiled schedule variants, radix multisets and their order, transforms
  * per block, elements per thread, one pass against two-pass splits, MFMA on
  * or off for 16-bit storage -- over `batch` transforms of synthetic data in
- * HBM (0: about 256 MB per launch), forward and inverse, and makes the
+ * HBM (0: about 1 GB per launch), forward and inverse, and makes the
  * fastest the plan that sfft_plan_create returns for that key from then on.
  * Tuned choices are saved / loaded as JSON keyed by the GPU architecture and
  * the HIP runtime version; a file from another device or runtime is ignored
