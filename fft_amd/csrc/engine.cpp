@@ -292,7 +292,12 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         if (cn_ > kMaxBluestein)
             throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "size " + std::to_string(n) + " too large");
         bs_m_ = bluestein_length(cn_);
-        bs_inner_.reset(new Plan(bs_m_, kC2C, prec));
+        try {
+            bs_inner_.reset(new Plan(bs_m_, kC2C, prec));
+        } catch (const Error &e) {
+            throw Error(e.code, "size " + std::to_string(n) + " needs a chirp-z length " +
+                                    std::to_string(bs_m_) + ": " + e.what());
+        }
         if (prec == kF64) {
             build_tables<double>();
             build_bluestein<double>();
@@ -363,7 +368,10 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
                 best_n1 = n1;
             }
         }
-        if (!best_n1) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "no two-pass split");
+        if (!best_n1)
+            throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
+                        "size " + std::to_string(n) + ": no two-pass split fits LDS (F64 plans "
+                        "reach 2^26, F32 2^28)");
         const size_t n1 = best_n1, n2 = cn_ / n1;
         PassPlan p1, p2;
         layout_columns((int)n1, prec, (int)std::min(n2, max_cols), nullptr, p1);

@@ -99,10 +99,13 @@ size_t sfft_size_maximum(size_t n);
  * including the reference's quirks (sizeMaximum may exceed n). */
 size_t sfft_real_size_minimum(size_t n);
 size_t sfft_real_size_maximum(size_t n);
-/* 1 if the engine plans this length for the kind (F32 / F64), else 0.  Every
+/* 1 if the engine plans this length for the kind in F32, else 0.  Every
  * length up to 2^27 is planned, as the reference accepts every length
  * (signalsmith-fft.h:187-215 handles any factor): lengths whose factors are
- * all <= 13 run directly, the others as a chirp-z convolution. */
+ * all <= 13 run directly (up to 2^28), the others as a chirp-z convolution.
+ * F64 plans reach 2^26 directly and 2^25 by chirp-z (two-pass column tiles
+ * of 16-byte values must fit one CU's LDS); beyond, sfft_plan_create
+ * returns SFFT_ERR_UNSUPPORTED_SIZE. */
 int sfft_size_supported(size_t n, int kind);
 
 /* ---- plans ---- */

@@ -1,0 +1,156 @@
+"""Edge cases of the transform path: empty batches, the smallest and the
+largest lengths, strided batches (padding untouched), argument errors,
+empty uneven batches.  The largest lengths are checked through closed forms
+evaluated on the device (a single bin transforms to N at that bin)."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import fft_amd
+from fft_amd import _lib as L
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _sh():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _strided(plan, x, in_dist, y, out_dist, batch, direction):
+    L.check(L.lib().sfft_execute_strided(plan._h, ctypes.c_void_p(x.data_ptr()), in_dist,
+                                         ctypes.c_void_p(y.data_ptr()), out_dist, batch,
+                                         direction, _sh()))
+
+
+def test_zero_batch_is_a_no_op():
+    p = fft_amd.Plan(64, L.C2C, L.F32)
+    x = torch.ones(64, dtype=torch.complex64, device="cuda")
+    y = torch.full_like(x, 7)
+    p.execute_ptr(x.data_ptr(), y.data_ptr(), 0, 0, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.all(y == 7)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_smallest_lengths(n):
+    x = (np.arange(n) + 1 + 1j * np.arange(n)).astype(np.complex128)
+    for inv in (False, True):
+        f = fft_amd.FFT(n, dtype=torch.complex128)
+        xd = torch.from_numpy(x).cuda()
+        yd = torch.empty_like(xd)
+        (f.ifft if inv else f.fft)(xd, yd)
+        torch.cuda.synchronize()
+        assert po.rel_l2(yd.cpu().numpy(), po.dft_ld(x, 1 if inv else -1)) < 1e-14
+
+
+def test_smallest_real_length():
+    rf = fft_amd.RealFFT(2, dtype=torch.float64)
+    x = torch.tensor([[3.0, 5.0]], dtype=torch.float64, device="cuda")
+    y = torch.empty(1, 1, dtype=torch.complex128, device="cuda")
+    rf.fft(x, y)
+    torch.cuda.synchronize()
+    assert complex(y[0, 0]) == complex(8.0, -2.0)  # (DC, Nyquist) packed in bin 0
+
+
+@pytest.mark.parametrize("n", [4096, 1920, 777, 40000, 17])
+def test_strided_batches_leave_padding_untouched(n):
+    """sfft_execute_strided with distances larger than the length (fixed-plan,
+    runtime-planned, two-pass and chirp-z lengths)."""
+    batch, din, dout = 5, n + 37, n + 11
+    rng = np.random.default_rng(n)
+    xa = (rng.standard_normal((batch, din)) + 1j * rng.standard_normal((batch, din))).astype(np.complex64)
+    x = torch.from_numpy(xa).cuda()
+    y = torch.full((batch, dout), complex(9.0, -9.0), dtype=torch.complex64, device="cuda")
+    p = fft_amd.Plan(n, L.C2C, L.F32)
+    for direction in (0, 1):
+        _strided(p, x, din, y, dout, batch, direction)
+        torch.cuda.synchronize()
+        got = y.cpu().numpy()
+        want = po.fft(xa[:, :n].astype(np.complex128), inverse=bool(direction))
+        assert po.rel_l2(got[:, :n], want) <= 1e-6, (n, direction)
+        assert np.all(got[:, n:] == np.complex64(9 - 9j))
+
+
+def test_strided_real_batches():
+    n, batch = 2048, 3
+    din, dout = n + 6, n // 2 + 5  # reals in, complex out
+    xr = torch.randn(batch, din, dtype=torch.float32, device="cuda")
+    y = torch.full((batch, dout), complex(9.0, 9.0), dtype=torch.complex64, device="cuda")
+    p = fft_amd.Plan(n, L.R2C, L.F32)
+    _strided(p, xr, din, y, dout, batch, 0)
+    torch.cuda.synchronize()
+    want = po.real_fft(xr[:, :n].double().cpu().numpy())
+    got = y.cpu().numpy()
+    assert po.rel_l2(got[:, :n // 2], want) <= 1e-6
+    assert np.all(got[:, n // 2:] == np.complex64(9 + 9j))
+
+
+def test_argument_errors():
+    p = fft_amd.Plan(64, L.C2C, L.F32)
+    x = torch.zeros(4, 64, dtype=torch.complex64, device="cuda")
+    y = torch.zeros_like(x)
+    lib = L.lib()
+    rc = lib.sfft_execute(p._h, None, ctypes.c_void_p(y.data_ptr()), 1, 0, None)
+    assert rc == L.SFFT_ERR_INVALID_ARG
+    rc = lib.sfft_execute(p._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), 1, 7,
+                          None)
+    assert rc == L.SFFT_ERR_INVALID_ARG
+    rc = lib.sfft_execute_strided(p._h, ctypes.c_void_p(x.data_ptr()), 32,
+                                  ctypes.c_void_p(y.data_ptr()), 64, 2, 0, None)
+    assert rc == L.SFFT_ERR_INVALID_ARG  # distance shorter than a transform
+    # overlapping strided ranges: output starts inside the input's last transform
+    rc = lib.sfft_execute_strided(p._h, ctypes.c_void_p(x.data_ptr()), 64,
+                                  ctypes.c_void_p(x.data_ptr() + 8 * 100), 64, 2, 0, None)
+    assert rc == L.SFFT_ERR_ALIASING
+    assert lib.sfft_last_error()
+    with pytest.raises(L.SfftError):
+        fft_amd.Plan(0, L.C2C, L.F32)
+    with pytest.raises(L.SfftError):
+        fft_amd.Plan(63, L.R2C, L.F32)  # odd real length
+    with pytest.raises(L.SfftError):
+        fft_amd.Plan(64, 9, L.F32)
+    with pytest.raises(L.SfftError):
+        fft_amd.Plan(64, L.C2C, 9)
+    with pytest.raises(L.SfftError) as e:  # F64 two-pass columns fit LDS up to 2^26
+        fft_amd.Plan(1 << 27, L.C2C, L.F64)
+    assert e.value.code == L.SFFT_ERR_UNSUPPORTED_SIZE
+
+
+def test_empty_uneven_batch():
+    b = fft_amd.UnevenBatch([], [])
+    xin = torch.zeros(16, dtype=torch.uint8, device="cuda")
+    out = torch.zeros(16, dtype=torch.uint8, device="cuda")
+    b.execute(xin, out)
+    b.execute_per_size(xin, out)
+    torch.cuda.synchronize()
+
+
+def _single_bin(n, b, dtype, cdtype):
+    k = torch.arange(n, device="cuda", dtype=torch.int64)
+    ph = ((k * b) % n).to(torch.float64) * (2 * np.pi / n)
+    return torch.polar(torch.ones_like(ph), ph).to(cdtype)
+
+
+@pytest.mark.parametrize("n,prec", [(1 << 28, L.F32), ((1 << 25) - 1, L.F64)],
+                         ids=["2^28_f32_two_pass", "2^25-1_f64_chirp_z"])
+def test_largest_lengths_single_bin(n, prec):
+    """e^{2 pi i b k / N} -> N delta_b; inverse of N delta_b -> N e^{...}."""
+    cdt = torch.complex64 if prec == L.F32 else torch.complex128
+    b = 12345
+    x = _single_bin(n, b, None, cdt)
+    y = torch.empty_like(x)
+    p = fft_amd.Plan(n, L.C2C, prec)
+    p.execute_ptr(x.data_ptr(), y.data_ptr(), 1, 0, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    peak = complex(y[b])
+    y[b] = 0
+    rest = float(torch.linalg.vector_norm(y.to(torch.complex128)))
+    # rel L2 against the exact N delta_b (whose norm is N)
+    rel = np.hypot(abs(peak - n), rest) / n
+    print(n, "rel L2", rel)
+    assert rel < (1e-6 if prec == L.F32 else 1e-12), rel
+    del x, y
+    torch.cuda.empty_cache()
