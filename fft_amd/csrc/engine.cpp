@@ -234,6 +234,7 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     int ept = 0;
     for (int e : {kEptSmall, kEptLarge}) {
         if (prec == kF64 && e > kEptSmall) continue;
+        if (h && h->ept && h->ept != e) continue;
         if (choose_radices(n, e, radices, prec == kF64 ? 16 : 25) &&
             threads_for(n, radices, e) <= max_block(prec)) {
             ept = e;
@@ -241,7 +242,6 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
         }
     }
     if (!ept) return false;
-    (void)h;
     const int tpt = threads_for(n, radices, ept);
     const int stride = lds_padded(n);
     const size_t slot_lds = (size_t)stride * eb;
@@ -355,10 +355,8 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
             if (h && h->split > 1 && n1 != h->split) continue;
             const size_t n2 = cn_ / n1;
             PassPlan p1, p2;
-            if (!layout_columns((int)n1, prec, (int)std::min(n2, max_cols), nullptr, p1))
-                continue;
-            if (!layout_columns((int)n2, prec, (int)std::min(n1, max_cols), nullptr, p2))
-                continue;
+            if (!layout_columns((int)n1, prec, (int)std::min(n2, max_cols), h, p1)) continue;
+            if (!layout_columns((int)n2, prec, (int)std::min(n1, max_cols), h, p2)) continue;
             if (p1.pd.nstages < 2) continue;
             // prefer wide column tiles (coalescing), then balance
             const double score = -std::log2((double)std::min(p1.pd.tpw, p2.pd.tpw)) * 4.0 +
@@ -374,8 +372,8 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
                         "reach 2^26, F32 2^28)");
         const size_t n1 = best_n1, n2 = cn_ / n1;
         PassPlan p1, p2;
-        layout_columns((int)n1, prec, (int)std::min(n2, max_cols), nullptr, p1);
-        layout_columns((int)n2, prec, (int)std::min(n1, max_cols), nullptr, p2);
+        layout_columns((int)n1, prec, (int)std::min(n2, max_cols), h, p1);
+        layout_columns((int)n2, prec, (int)std::min(n1, max_cols), h, p2);
         p1.slots_per_transform = (int)n2;
         p1.in_slot_stride = 1;
         p1.in_elem_stride = (int64_t)n2;
@@ -824,7 +822,9 @@ std::vector<PlanHints> plan_candidates(size_t n, int kind, int prec) {
     }
     std::stable_sort(splits.begin(), splits.end());
     for (size_t i = 0; i < std::min<size_t>(splits.size(), 3); ++i) {
-        for (int cols : {0, 16, 4}) {  // column tile width (0: the planner's, 8)
+        // column tile width (0: the planner's, 8); 32 elements per thread
+        // measured slower on every column pass, so it is not offered
+        for (int cols : {0, 16, 4}) {
             PlanHints h;
             h.split = splits[i].second;
             h.tpw = cols;
