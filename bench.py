@@ -234,17 +234,21 @@ def precision_sweep(torch, fft_amd, reps=5):
     (32 N bytes FP64, 16 N FP32, 8 N 16-bit)."""
     from fft_amd import _lib as L
     sh = torch.cuda.current_stream().cuda_stream
-    out = {}
+    out = {"note": "f16 / bf16 store 16-bit values and run float butterflies; the "
+                   "*_mfma_radix16 rows run the radix-16 stages on MFMA (split hi+lo operands)"}
     for n, batch, seed in ((1 << 20, 16, 20), (1 << 24, 1, 24)):
         g = torch.Generator(device="cuda").manual_seed(seed)
         base = torch.randn(batch, n, 2, device="cuda", generator=g)
         row = {}
-        for name, prec, dt, eb in (("f64", L.F64, torch.float64, 8), ("f32", L.F32, torch.float32, 4),
-                                   ("f16", L.F16, torch.float16, 2),
-                                   ("bf16", L.BF16, torch.bfloat16, 2)):
+        for name, prec, dt, eb, mfma in (
+                ("f64", L.F64, torch.float64, 8, -1), ("f32", L.F32, torch.float32, 4, -1),
+                ("f16", L.F16, torch.float16, 2, -1), ("bf16", L.BF16, torch.bfloat16, 2, -1),
+                ("f16_mfma_radix16", L.F16, torch.float16, 2, 1),
+                ("bf16_mfma_radix16", L.BF16, torch.bfloat16, 2, 1)):
             xs = base.to(dt).contiguous()
             ys = torch.empty_like(xs)
-            plan = fft_amd.Plan(n, L.C2C, prec)
+            plan = (fft_amd.Plan(n, L.C2C, prec) if mfma < 0 else
+                    fft_amd.Plan(n, L.C2C, prec, mfma=mfma))
             plan.execute_ptr(xs.data_ptr(), ys.data_ptr(), batch, L.FORWARD, sh)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -267,7 +267,7 @@ static void enable_mfma(PassPlan &pp) {
     const int n = pp.pd.n;
     bool has16 = false;
     for (int s = 0; s < pp.pd.nstages; ++s) has16 |= pp.pd.st[s].radix == 16;
-    if (!has16 || n % 16) return;
+    if (!has16 || n % 16 || pp.ept != kEptSmall) return;
     const int nf = pp.pd.tpw * (n / 16);
     const int waves = (int)(pp.block / 64);
     pp.pd.mfma = (pp.block % 64 == 0) && (nf % 32 == 0) && (nf / 32 <= waves * (pp.ept / 8));
@@ -393,7 +393,10 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         passes_.push_back(p1);
         passes_.push_back(p2);
     }
-    if (prec >= kF16 && !(h && h->mfma == 0)) {
+    // 16-bit storage: radix-16 stages on MFMA when asked for (measured slower
+    // than float butterflies here: the operand splits and the larger register
+    // footprint outweigh the matrix-core arithmetic in an HBM-bound pass)
+    if (prec >= kF16 && h && h->mfma == 1) {
         for (auto &pp : passes_) enable_mfma(pp);
     }
     if (prec == kF64)
@@ -770,7 +773,7 @@ std::vector<PlanHints> plan_candidates(size_t n, int kind, int prec) {
     }
     if (prec >= kF16) {
         PlanHints h;
-        h.mfma = 0;
+        h.mfma = 1;
         out.push_back(h);
     }
     const int maxr = prec == kF64 ? 16 : 25;

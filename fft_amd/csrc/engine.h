@@ -68,7 +68,7 @@ struct PlanHints {
     int ept = 0;               // elements-per-thread instance (0: planner's)
     size_t split = 0;          // N1 of a two-pass plan (0: planner's; 1: force single pass)
     int fixed_variant = -1;    // -1 default, -2 no fixed kernel, k >= 0 that variant
-    int mfma = -1;             // 16-bit storage: -1 default (on), 0 scalar radix-16 butterflies
+    int mfma = -1;             // 16-bit storage: 1 radix-16 stages on MFMA; -1 / 0 float butterflies
 };
 
 class Plan {

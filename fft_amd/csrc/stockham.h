@@ -105,7 +105,7 @@ __device__ __forceinline__ C2<T> real_pre_bin(const C2<T> *x, const C2<T> *tmi, 
 // output offset in elements instead of the strided address map.
 // S is the HBM storage format of in / out (storage.h); with a 16-bit format
 // the pointers address 4-byte values and only complex-to-complex modes run.
-template <typename T, int EPT, typename S = StoreNative>
+template <typename T, int EPT, typename S = StoreNative, bool MFMA = false>
 struct Block {
     using St = Storage<T, S>;
     const PassArgs<T> &a;
@@ -193,7 +193,7 @@ struct Block {
     // One Stockham stage of radix R.
     template <int R>
     __device__ __forceinline__ void stage(int s, bool src_global, bool dst_global, bool col_map) {
-        if constexpr (R == 16 && !St::native) {
+        if constexpr (R == 16 && !St::native && MFMA) {
             if (mfma_) {
                 stage16_mfma(s, src_global, dst_global, col_map);
                 return;
@@ -329,11 +329,13 @@ struct Block {
     // tpw * n/16 butterflies are flattened (slot-major, or slot-minor for
     // column-strided HBM access) and taken 32 at a time by a wave: lane l
     // holds points mfma_point(i, l / 32) of butterfly l % 32 of its group.
-    // The host enables it only when blockDim is a multiple of 64, the
-    // butterfly count a multiple of 32 and at most EPT / 8 groups per wave.
+    // Compiled only into the MFMA instances (the 16-element one: 32 elements
+    // would spill); the host enables it when blockDim is a multiple of 64,
+    // the butterfly count a multiple of 32 and at most EPT / 8 groups per
+    // wave.
     __device__ __forceinline__ void stage16_mfma(int s, bool src_global, bool dst_global,
                                                  bool col_map) {
-        if constexpr (!St::native) {
+        if constexpr (!St::native && MFMA) {
             constexpr int GM = EPT / 8;
             const StageDesc st = a.pd.st[s];
             const int nb = n_ / 16, ns = st.ns;
@@ -347,6 +349,8 @@ struct Block {
                 const int gi = wv + q * nw;
                 gslot[q] = 0;
                 gj[q] = 0;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[q][i] = {(T)0, (T)0};
                 if (gi < ngroups) {  // wave-uniform
                     const int f = gi * 32 + col;
                     const int slot = col_map ? f % tpw_ : f / nb;
@@ -386,9 +390,13 @@ struct Block {
                             if (r) v[q][i] = cmul(v[q][i], w[r - 1]);
                         }
                     }
-                    dft16_mfma<S>(v[q], dm);
                 }
             }
+            // every group, active or not (inactive ones hold zeros and are
+            // never stored), two at a time: their MFMAs interleave
+#pragma unroll
+            for (int q = 0; q < GM; q += 2)
+                dft16_mfma<S, 2>(*reinterpret_cast<C2<T>(*)[2][8]>(&v[q]), dm);
 #pragma unroll
             for (int q = 0; q < GM; ++q) {
                 if (wv + q * nw < ngroups) {

@@ -13,8 +13,9 @@
 // in float.  Operands are split into two 16-bit parts (x = hi + lo, both for
 // the data and for the DFT matrix) and the three products hi*hi, hi*lo,
 // lo*hi are accumulated, which keeps the contraction near float accuracy
-// (the lo*lo term is below 2^-21 of the result).  Each column is scaled by a
-// power of two before the split so that half's range is never exceeded.
+// (the lo*lo term is below 2^-21 of the result).  With IEEE half each
+// column is scaled by a power of two before the split so that half's range
+// is never exceeded; bfloat16 has float's exponent range and needs none.
 //
 // Lane layout of v_mfma_f32_32x32x8 (checked on gfx950): lane l holds
 // A[l % 32][4 (l / 32) + i] and B[4 (l / 32) + i][l % 32] for i < 4, and the
@@ -80,22 +81,31 @@ struct Storage<float, StoreBF16> {
         return {__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
     }
     __device__ static void store(void *p, int64_t i, C2<float> v) {
-        static_cast<uint32_t *>(p)[i] = f32_to_bf16(v.x) | (f32_to_bf16(v.y) << 16);
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        const b2 w = {(__bf16)v.x, (__bf16)v.y};  // one packed round-to-nearest-even convert
+        static_cast<uint32_t *>(p)[i] = __builtin_bit_cast(uint32_t, w);
     }
 };
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
+// split2(a, b): the 16-bit hi parts of a and b packed in one dword (a in the
+// low half) and the 16-bit parts of the residuals a - hi, b - hi likewise.
+// The residual of a 16-bit rounding is exact in float, so x = hi + lo up to
+// the rounding of lo (2^-11 / 2^-8 of lo, about 2^-22 / 2^-16 of x).
 template <typename S>
 struct Mfma16;
 
 template <>
 struct Mfma16<StoreF16> {
     typedef _Float16 v4 __attribute__((ext_vector_type(4)));
-    typedef _Float16 elem;
-    __device__ static void split(float x, elem &hi, elem &lo) {
-        hi = (_Float16)x;
-        lo = (_Float16)(x - (float)hi);
+    static constexpr bool kScale = true;  // half's range: scale each column
+    __device__ static void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
+        const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
+        const auto l = __builtin_amdgcn_cvt_pkrtz(a - (float)h[0], b - (float)h[1]);
+        hi = __builtin_bit_cast(uint32_t, h);
+        lo = __builtin_bit_cast(uint32_t, l);
     }
     __device__ static f32x16 mma(v4 a, v4 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, c, 0, 0, 0);
@@ -105,16 +115,25 @@ struct Mfma16<StoreF16> {
 template <>
 struct Mfma16<StoreBF16> {
     typedef short v4 __attribute__((ext_vector_type(4)));
-    typedef short elem;
-    __device__ static void split(float x, elem &hi, elem &lo) {
-        const uint32_t h = f32_to_bf16(x);
-        hi = (short)h;
-        lo = (short)f32_to_bf16(x - __uint_as_float(h << 16));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    static constexpr bool kScale = false;  // float's exponent range
+    __device__ static void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
+        const b2 h = {(__bf16)a, (__bf16)b};
+        const uint32_t hw = __builtin_bit_cast(uint32_t, h);
+        const b2 l = {(__bf16)(a - __uint_as_float(hw << 16)),
+                      (__bf16)(b - __uint_as_float(hw & 0xffff0000u))};
+        hi = hw;
+        lo = __builtin_bit_cast(uint32_t, l);
     }
     __device__ static f32x16 mma(v4 a, v4 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
     }
 };
+
+template <typename S>
+__device__ __forceinline__ typename Mfma16<S>::v4 as_v4(uint32_t w0, uint32_t w1) {
+    return __builtin_bit_cast(typename Mfma16<S>::v4, (u32x2){w0, w1});
+}
 
 // cos(2 pi t / 16), t = 0..15.
 static __constant__ float kCos16[16] = {
@@ -131,67 +150,93 @@ __device__ __forceinline__ int mfma_point(int i, int h) { return (i < 4) ? 4 * h
 template <typename S>
 struct Dft16Operands {
     typename Mfma16<S>::v4 hi[4], lo[4];
-    __device__ Dft16Operands(int lane) {
+    __device__ static float entry(int m, int c) {
+        const int mm = m & 15, cc = c & 15;
+        const bool out_im = m >= 16, in_im = c >= 16;
+        const int t = (mm * cc) & 15;
+        const float cs = kCos16[t], sn = kCos16[(t + 12) & 15];  // sin(x) = cos(x - pi/2)
+        return (out_im == in_im) ? cs : (in_im ? sn : -sn);     // [[C, S], [-S, C]]
+    }
+    __device__ explicit Dft16Operands(int lane) {
         const int m = lane & 31, h = lane >> 5;
-        const int mm = m & 15;
-        const bool out_im = m >= 16;
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int c = 8 * ks + 4 * h + i;
-                const int cc = c & 15;
-                const bool in_im = c >= 16;
-                const int t = (mm * cc) & 15;
-                const float cs = kCos16[t], sn = kCos16[(t + 12) & 15];  // sin(x) = cos(x - pi/2)
-                // [[C, S], [-S, C]]
-                const float g = (out_im == in_im) ? cs : (in_im ? sn : -sn);
-                typename Mfma16<S>::elem a, b;
-                Mfma16<S>::split(g, a, b);
-                hi[ks][i] = a;
-                lo[ks][i] = b;
-            }
+            const int c0 = 8 * ks + 4 * h;
+            uint32_t h0, l0, h1, l1;
+            Mfma16<S>::split2(entry(m, c0), entry(m, c0 + 1), h0, l0);
+            Mfma16<S>::split2(entry(m, c0 + 2), entry(m, c0 + 3), h1, l1);
+            hi[ks] = as_v4<S>(h0, h1);
+            lo[ks] = as_v4<S>(l0, l1);
         }
     }
 };
 
-// Forward 16-point DFTs of the 32 columns held by a wave (8 points per lane,
-// local order of mfma_point), in place.
-template <typename S>
-__device__ __forceinline__ void dft16_mfma(C2<float> (&v)[8], const Dft16Operands<S> &g) {
+// Forward 16-point DFTs of G groups of 32 columns held by a wave (8 points
+// per lane and group, local order of mfma_point), in place.  K steps: ks 0 =
+// re of local points 0..3, 1 = re of 4..7, 2 = im of 0..3, 3 = im of 4..7
+// (rows 8 ks + 4 h + i).  The MFMAs of the groups are interleaved, and the
+// hi*hi products and the two correction products accumulate separately, so
+// 2 G independent accumulation chains are in flight.
+template <typename S, int G>
+__device__ __forceinline__ void dft16_mfma(C2<float> (&v)[G][8], const Dft16Operands<S> &g) {
     using M = Mfma16<S>;
-    float mx = 0.f;
+    int ex[G];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) mx = fmaxf(mx, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
-    mx = fmaxf(mx, __shfl_xor(mx, 32));
-    int ex = 0;
-    (void)frexpf(mx, &ex);  // mx < 2^ex: the scaled column lies in (-1, 1)
-    typename M::v4 bh[4], bl[4];
+    for (int q = 0; q < G; ++q) {
+        ex[q] = 0;
+        if constexpr (M::kScale) {
+            float mx = 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        typename M::elem h, l;
-        M::split(ldexpf(v[i].x, -ex), h, l);
-        bh[0][i] = h;
-        bl[0][i] = l;
-        M::split(ldexpf(v[4 + i].x, -ex), h, l);
-        bh[1][i] = h;
-        bl[1][i] = l;
-        M::split(ldexpf(v[i].y, -ex), h, l);
-        bh[2][i] = h;
-        bl[2][i] = l;
-        M::split(ldexpf(v[4 + i].y, -ex), h, l);
-        bh[3][i] = h;
-        bl[3][i] = l;
+            for (int i = 0; i < 8; ++i) mx = fmaxf(mx, fmaxf(fabsf(v[q][i].x), fabsf(v[q][i].y)));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            (void)frexpf(mx, &ex[q]);  // mx < 2^ex: the scaled column lies in (-1, 1)
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                v[q][i] = {ldexpf(v[q][i].x, -ex[q]), ldexpf(v[q][i].y, -ex[q])};
+        }
     }
-    f32x16 acc = {};
+    typename M::v4 bh[G][4], bl[G][4];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        uint32_t h[4][2], l[4][2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            M::split2(v[q][2 * p].x, v[q][2 * p + 1].x, h[0][p], l[0][p]);
+            M::split2(v[q][4 + 2 * p].x, v[q][5 + 2 * p].x, h[1][p], l[1][p]);
+            M::split2(v[q][2 * p].y, v[q][2 * p + 1].y, h[2][p], l[2][p]);
+            M::split2(v[q][4 + 2 * p].y, v[q][5 + 2 * p].y, h[3][p], l[3][p]);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            bh[q][ks] = as_v4<S>(h[ks][0], h[ks][1]);
+            bl[q][ks] = as_v4<S>(l[ks][0], l[ks][1]);
+        }
+    }
+    f32x16 acc[G], cor[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        acc[q] = (f32x16){};
+        cor[q] = (f32x16){};
+    }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-        acc = M::mma(g.hi[ks], bh[ks], acc);
-        acc = M::mma(g.hi[ks], bl[ks], acc);
-        acc = M::mma(g.lo[ks], bh[ks], acc);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            acc[q] = M::mma(g.hi[ks], bh[q][ks], acc[q]);
+            cor[q] = M::mma(g.hi[ks], bl[q][ks], cor[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) cor[q] = M::mma(g.lo[ks], bh[q][ks], cor[q]);
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = {ldexpf(acc[i], ex), ldexpf(acc[8 + i], ex)};
+    for (int q = 0; q < G; ++q) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            v[q][i] = {acc[q][i] + cor[q][i], acc[q][8 + i] + cor[q][8 + i]};
+            if constexpr (M::kScale)
+                v[q][i] = {ldexpf(v[q][i].x, ex[q]), ldexpf(v[q][i].y, ex[q])};
+        }
+    }
 }
 
 }  // namespace sfft

@@ -55,8 +55,9 @@ extern "C" {
 #define SFFT_F64 1 /* FFT<double> */
 /* 16-bit HBM storage, float arithmetic, complex-to-complex only (no
  * counterpart in the reference): complex values are interleaved IEEE half /
- * bfloat16 pairs (4 bytes); radix-16 stages run on the matrix cores with
- * split hi+lo operands and float accumulation. */
+ * bfloat16 pairs (4 bytes).  With the plan hint mfma = 1 the radix-16 stages
+ * run on the matrix cores with split hi+lo operands and float accumulation;
+ * by default they run as float butterflies (faster on MI355X). */
 #define SFFT_F16 2
 #define SFFT_BF16 3
 
@@ -128,7 +129,8 @@ typedef struct {
     int transforms_per_block; /* 0: planner's choice */
     size_t split;         /* two-pass N1 (0: planner's choice, 1: force one pass) */
     int elements_per_thread;  /* 16 or 32 (0: planner's choice) */
-    int mfma;             /* 16-bit storage: -1 default (radix-16 stages on MFMA), 0 scalar */
+    int mfma;             /* 16-bit storage: 1 = radix-16 stages on MFMA; -1 (default) or
+                             0 = float butterflies (measured faster) */
 } sfft_plan_hints;
 
 /* A plan built with hints (not cached, not used by sfft_plan_create). */
@@ -140,8 +142,8 @@ int sfft_plan_variants(size_t n, int kind, int precision);
 /* ---- autotuner (no counterpart in the reference) ----
  * sfft_autotune times the planner's candidates for (n, kind, precision) --
  * compiled schedule variants, radix multisets and their order, transforms
- * per block, elements per thread, one pass against two-pass splits, MFMA on
- * or off for 16-bit storage -- over `batch` transforms of synthetic data in
+ * per block, elements per thread, two-pass splits and column tile widths,
+ * MFMA radix-16 stages for 16-bit storage -- over `batch` transforms of synthetic data in
  * HBM (0: about 1 GB per launch), forward and inverse, and makes the
  * fastest the plan that sfft_plan_create returns for that key from then on.
  * Tuned choices are saved / loaded as JSON keyed by the GPU architecture and

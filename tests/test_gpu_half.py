@@ -1,5 +1,6 @@
-"""16-bit storage variant (IEEE half / bfloat16 in HBM, float arithmetic,
-radix-16 stages on MFMA with split hi+lo operands).
+"""16-bit storage variant (IEEE half / bfloat16 in HBM, float arithmetic),
+with radix-16 stages as float butterflies (default) or on MFMA with split
+hi+lo operands (plan hint mfma = 1).
 
 The reference has no such variant; it is checked against the FP64 oracle on
 the same (16-bit) input values.  Besides a plain tolerance, the error is
@@ -24,12 +25,12 @@ NAME = {L.F16: "f16", L.BF16: "bf16"}
 TOL = {L.F16: 1.5e-3, L.BF16: 1.2e-2}
 
 
-def _run(n, batch, prec, direction, seed=1, plan=None):
+def _run(n, batch, prec, direction, seed=1, mfma=-1):
     g = torch.Generator(device="cpu").manual_seed(seed + n)
     x32 = torch.randn(batch, n, 2, generator=g)
     xs = x32.to(DT[prec]).cuda()
     ys = torch.empty_like(xs)
-    p = plan or fft_amd.Plan(n, L.C2C, prec)
+    p = fft_amd.Plan(n, L.C2C, prec) if mfma < 0 else fft_amd.Plan(n, L.C2C, prec, mfma=mfma)
     p.execute_ptr(xs.data_ptr(), ys.data_ptr(), batch, direction,
                   torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
@@ -46,14 +47,17 @@ def _round_to(z, prec):
     return t[..., 0] + 1j * t[..., 1]
 
 
+@pytest.mark.parametrize("mfma", [1, -1], ids=["mfma", "butterflies"])
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
 @pytest.mark.parametrize("n,batch", [(256, 64), (1024, 32), (4096, 16), (16384, 4), (8192, 8)])
 @pytest.mark.parametrize("direction", [0, 1], ids=["fwd", "inv"])
-def test_single_pass_pow2(prec, n, batch, direction):
-    p, x, y = _run(n, batch, prec, direction)
+def test_single_pass_pow2(prec, n, batch, direction, mfma):
+    p, x, y = _run(n, batch, prec, direction, mfma=mfma)
     info = p.info()
     assert info["passes"] == 1
-    assert info["mfma_stages"] >= 1, info
+    # MFMA stages run in the 16-element instance (<= 512 threads per
+    # transform): up to 8192 in one pass
+    assert (info["mfma_stages"] >= 1) == (mfma == 1 and n <= 8192), info
     want = po.fft(x, inverse=bool(direction))
     err = po.rel_l2(y, want)
     ideal = po.rel_l2(_round_to(want, prec), want)
@@ -62,12 +66,13 @@ def test_single_pass_pow2(prec, n, batch, direction):
     assert err <= 1.3 * ideal + 1e-6, (err, ideal)
 
 
+@pytest.mark.parametrize("mfma", [1, -1], ids=["mfma", "butterflies"])
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
 @pytest.mark.parametrize("n,batch", [(1 << 16, 2), (1 << 20, 1)])
-def test_two_pass_pow2(prec, n, batch):
-    p, x, y = _run(n, batch, prec, 0, seed=3)
+def test_two_pass_pow2(prec, n, batch, mfma):
+    p, x, y = _run(n, batch, prec, 0, seed=3, mfma=mfma)
     info = p.info()
-    assert info["passes"] == 2 and info["mfma_stages"] >= 1, info
+    assert info["passes"] == 2 and (info["mfma_stages"] >= 1) == (mfma == 1), info
     want = po.fft(x)
     err = po.rel_l2(y, want)
     ideal = po.rel_l2(_round_to(want, prec), want)
@@ -87,11 +92,12 @@ def test_mixed_sizes(prec, n, batch):
     assert err <= TOL[prec]
 
 
+@pytest.mark.parametrize("mfma", [1, -1], ids=["mfma", "butterflies"])
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
-def test_2e24_against_fp64_gpu(prec):
+def test_2e24_against_fp64_gpu(prec, mfma):
     """N = 2^24: the FP64 GPU plan (itself checked against the oracle) is the reference."""
     n = 1 << 24
-    p, x, y = _run(n, 1, prec, 0, seed=7)
+    p, x, y = _run(n, 1, prec, 0, seed=7, mfma=mfma)
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
     fft_amd.Plan(n, L.C2C, L.F64).execute_ptr(xd.data_ptr(), yd.data_ptr(), 1, 0,
