@@ -41,7 +41,7 @@ void UnevenBatch::build(size_t lds_cap) {
     // One descriptor set per (plan, direction); the passes and radices are
     // the plan's own, so every transform computes exactly what sfft_execute
     // computes for it.  Only the slots per round change (tpw): the block is
-    // kPersistentBlock threads wide.
+    // persistent_block<T>() threads wide.
     std::vector<PassArgs<T>> table;
     std::map<std::pair<const Plan *, int>, int> first_entry;
     size_t lds_need = 0;
@@ -65,7 +65,7 @@ void UnevenBatch::build(size_t lds_cap) {
                 PassArgs<T> a = p.pass_args<T>(q, inv);
                 const size_t sl = p.slot_lds_bytes(q);
                 if (p.passes().size() == 1) {
-                    int tpw = kPersistentBlock / a.pd.tpt;
+                    int tpw = persistent_block<T>() / a.pd.tpt;
                     if (sl) tpw = std::min<int>(tpw, (int)(lds_cap / sl));
                     a.pd.tpw = std::max(1, tpw);
                 } else {
@@ -73,7 +73,7 @@ void UnevenBatch::build(size_t lds_cap) {
                     // block and LDS allow; the last round of a transform masks
                     // the columns past its end
                     const int spt = p.passes()[q].slots_per_transform;
-                    int tpw = kPersistentBlock / a.pd.tpt;
+                    int tpw = persistent_block<T>() / a.pd.tpt;
                     if (sl) tpw = std::min<int>(tpw, (int)(lds_cap / sl));
                     tpw = std::min(tpw, spt);
                     int c = 1;

@@ -93,13 +93,13 @@ static double radix_cost(int n, const std::vector<int> &radices, int ept, int tp
 }
 
 static void search_radices(int rem, int max_r, std::vector<int> &cur, int n, int ept,
-                           double &best_cost, std::vector<int> &best) {
+                           int max_threads, double &best_cost, std::vector<int> &best) {
     // 25 complex values exceed the register budget of the 16-element instance
     // in FP64 and of small per-thread budgets: only offered when ept >= 16 and
     // the caller allows it (max_r).
     if (rem == 1) {
         const int tpt = threads_for(n, cur, ept);
-        if (tpt > 1024) return;
+        if (tpt > max_threads) return;
         const double c = radix_cost(n, cur, ept, tpt);
         if (c < best_cost - 1e-9) {
             best_cost = c;
@@ -111,12 +111,12 @@ static void search_radices(int rem, int max_r, std::vector<int> &cur, int n, int
     for (int r : kRadixSet) {
         if (r > max_r || rem % r) continue;
         cur.push_back(r);
-        search_radices(rem / r, r, cur, n, ept, best_cost, best);
+        search_radices(rem / r, r, cur, n, ept, max_threads, best_cost, best);
         cur.pop_back();
     }
 }
 
-bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix) {
+bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix, int max_threads) {
     radices.clear();
     if (n < 1) return false;
     if (n == 1) {
@@ -125,7 +125,7 @@ bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix) {
     }
     std::vector<int> cur;
     double best_cost = 1e30;
-    search_radices(n, max_radix, cur, n, ept, best_cost, radices);
+    search_radices(n, max_radix, cur, n, ept, max_threads, best_cost, radices);
     return !radices.empty();
 }
 
@@ -154,7 +154,9 @@ static size_t bluestein_length(size_t n) {
 }
 
 static size_t elem_bytes(int prec) { return prec == kF64 ? 16 : 8; }
-static int max_block(int prec) { return prec >= kF16 ? kMaxBlock16 : 1024; }
+static int max_block(int prec) {
+    return prec >= kF16 ? kMaxBlock16 : (prec == kF64 ? kMaxBlock<double>() : kMaxBlock<float>());
+}
 
 // Dynamic LDS a layout may use: the device limit less room for the kernels'
 // static __shared__ variables.
@@ -196,8 +198,7 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
             if (prod != n) return false;
             radices = h->radices;
             if (threads_for(n, radices, e) > max_block(prec)) continue;
-        } else if (!choose_radices(n, e, radices, prec == kF64 ? 16 : 25) ||
-                   threads_for(n, radices, e) > max_block(prec)) {
+        } else if (!choose_radices(n, e, radices, prec == kF64 ? 16 : 25, max_block(prec))) {
             continue;
         }
         ept = e;
@@ -235,8 +236,7 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     for (int e : {kEptSmall, kEptLarge}) {
         if (prec == kF64 && e > kEptSmall) continue;
         if (h && h->ept && h->ept != e) continue;
-        if (choose_radices(n, e, radices, prec == kF64 ? 16 : 25) &&
-            threads_for(n, radices, e) <= max_block(prec)) {
+        if (choose_radices(n, e, radices, prec == kF64 ? 16 : 25, max_block(prec))) {
             ept = e;
             break;
         }

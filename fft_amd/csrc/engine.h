@@ -156,7 +156,8 @@ void drop_cached_plan(size_t n, int kind, int prec);
 std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec);
 
 // Radix choice for a single pass of length n (exposed for tests/tuning).
-bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix = 25);
+bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix = 25,
+                    int max_threads = 1024);
 bool supported_length(size_t n);
 // Any length a plan exists for in F32 / F64: direct, or chirp-z (up to 2^27).
 bool plannable_length(size_t n);

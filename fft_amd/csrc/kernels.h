@@ -11,6 +11,15 @@ namespace sfft {
 constexpr int kEptSmall = 16;
 constexpr int kEptLarge = 32;
 
+// Block size limit of the pass kernel: 1024 threads for float; 512 for
+// double, whose radix-16 butterflies and twiddles need more than 128
+// registers (a single pass holds at most ~9600 double values in LDS, so no
+// double plan needs more than 512 threads).
+template <typename T>
+constexpr int kMaxBlock() {
+    return sizeof(T) == 8 ? 512 : 1024;
+}
+
 template <typename T>
 hipError_t launch_pass(const PassArgs<T> &a, int ept, unsigned grid, unsigned block,
                        size_t lds_bytes, hipStream_t stream);

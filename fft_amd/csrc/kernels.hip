@@ -7,7 +7,7 @@ namespace sfft {
 // ---------------------------------------------------------------- passes --
 
 template <typename T, int EPT>
-__global__ void __launch_bounds__(1024) k_pass(PassArgs<T> a) {
+__global__ void __launch_bounds__(kMaxBlock<T>()) k_pass(PassArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
     Block<T, EPT> blk(a, a.in, a.out, a.total_slots, lds, (int)threadIdx.x,

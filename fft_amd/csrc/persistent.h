@@ -20,7 +20,12 @@ struct WorkUnit {
     int64_t out_off;
 };
 
-constexpr int kPersistentBlock = 1024;
+// Threads per persistent workgroup: 1024 for float; 512 for double, whose
+// butterflies need more than the 128 registers a 1024-thread block allows.
+template <typename T>
+constexpr int persistent_block() {
+    return sizeof(T) == 8 ? 512 : 1024;
+}
 // WorkUnit.npasses bit: forward real two-pass unit, post-process after pass 2
 constexpr int32_t kUnitRealPost = 0x100;
 

@@ -4,7 +4,7 @@
 namespace sfft {
 
 template <typename T, int EPT>
-__global__ void __launch_bounds__(kPersistentBlock)
+__global__ void __launch_bounds__(persistent_block<T>())
     k_persistent(const WorkUnit *__restrict__ units, int nunits,
                  const PassArgs<T> *__restrict__ table, const int64_t *__restrict__ gather_in,
                  const int64_t *__restrict__ gather_out, unsigned *counters,
@@ -87,7 +87,7 @@ hipError_t launch_persistent(const WorkUnit *units, int nunits, const PassArgs<T
         }
         attr = true;
     }
-    hipLaunchKernelGGL((k_persistent<T, E>), dim3(grid), dim3(kPersistentBlock), lds_bytes,
+    hipLaunchKernelGGL((k_persistent<T, E>), dim3(grid), dim3(persistent_block<T>()), lds_bytes,
                        stream, units, nunits, table, gather_in, gather_out, counters,
                        (const char *)in_arena,
                        (char *)out_arena);
@@ -99,7 +99,7 @@ int persistent_blocks_per_cu(size_t lds_bytes) {
     constexpr int E = persistent_ept<T>();
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)k_persistent<T, E>,
-                                                     kPersistentBlock, lds_bytes) != hipSuccess)
+                                                     persistent_block<T>(), lds_bytes) != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
 }
