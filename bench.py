@@ -225,6 +225,46 @@ def uneven_batch(torch, fft_amd, count=20000, reps=5):
     return res
 
 
+def single_transform(torch, fft_amd, reps=200):
+    """One FFT<float> / FFT<double> of N = 4096 plus its ifft round trip, U[-1,1]
+    inputs (seed 4096): device-resident latency (HIP events around fft + ifft,
+    median of `reps`) and the host-staged latency of the drop-in path (host
+    array in, host array out, wall clock), with the round-trip error."""
+    from fft_amd import _lib as L
+    from fft_amd import synth
+    n = 4096
+    out = {}
+    for name, prec, cdt in (("f32", L.F32, torch.complex64), ("f64", L.F64, torch.complex128)):
+        f = fft_amd.FFT(n, dtype=cdt)
+        xh = synth.complex_uniform(n, (n,), np.complex64 if prec == L.F32 else np.complex128)
+        x = torch.from_numpy(xh).cuda()
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            f.fft(x, y)
+            f.ifft(y, z)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e3)
+        rt = float(torch.linalg.vector_norm((z / n - x).to(torch.complex128)) /
+                   torch.linalg.vector_norm(x.to(torch.complex128)))
+        hs = []
+        yh = np.empty_like(xh)
+        zh = np.empty_like(xh)
+        for _ in range(20):
+            t0 = time.perf_counter()
+            f.fft(xh, yh)
+            f.ifft(yh, zh)
+            hs.append((time.perf_counter() - t0) * 1e6)
+        out[name] = {"device_round_trip_us": round(float(np.median(ts)), 2),
+                     "host_staged_round_trip_us": round(float(np.median(hs)), 1),
+                     "round_trip_rel_l2": float(f"{rt:.3e}")}
+    return out
+
+
 def precision_sweep(torch, fft_amd, reps=5):
     """Long 1D transforms N = 2^20 x 16 and 2^24 x 1, N(0,1) complex inputs
     from a seed: FP64, FP32 and the 16-bit storage variants (IEEE half,
@@ -428,6 +468,7 @@ def main():
         "wall_s_timed": round(wall, 3),
     }
     if not args.no_extras:
+        result["single_transform_4096"] = single_transform(torch, fft_amd)
         result["uneven_batch"] = uneven_batch(torch, fft_amd)
         result["precision_sweep"] = precision_sweep(torch, fft_amd)
         result["serving"] = serving()
