@@ -8,6 +8,9 @@ namespace sfft {
 namespace {
 const FixedC2C<float, 16384, 1024, 1, Sched<0, 0, 0, 1>, 0, 16, 16, 16, 4> c16384a;
 const FixedC2C<float, 16384, 1024, 1, Sched<-1, 0, 0>, 1, 16, 16, 16, 4> c16384b;
+// three stages of radix 32/32/16 on 512 threads (256 registers per thread)
+const FixedC2C<float, 16384, 512, 1, Sched<0, 0, 0, 1>, 2, 32, 32, 16> c16384c;
+const FixedC2C<float, 16384, 512, 1, Sched<-1, 0, 0>, 3, 32, 32, 16> c16384d;
 const FixedC2C<float, 8192, 512, 1, Sched<-1, 4, 0>, 0, 16, 16, 8, 4> c8192a;
 const FixedC2C<float, 8192, 512, 1, Sched<0, 4, 0>, 1, 16, 16, 8, 4> c8192b;
 }  // namespace
