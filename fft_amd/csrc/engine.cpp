@@ -525,9 +525,9 @@ void Plan::build_tables() {
         while ((1ull << (2 * shift)) < cn_) ++shift;
         ptw_shift_ = shift;
         const size_t nf = 1ull << shift, nc = (cn_ + nf - 1) >> shift;
-        alloc((void **)&d_ptw_fine_, nf * sizeof(C2<double>));
-        alloc((void **)&d_ptw_coarse_, nc * sizeof(C2<double>));
-        check_hip(gen_post_twiddles(d_ptw_fine_, d_ptw_coarse_, shift, cn_, s),
+        alloc(&d_ptw_fine_, nf * sizeof(C2<T>));
+        alloc(&d_ptw_coarse_, nc * sizeof(C2<T>));
+        check_hip(gen_post_twiddles<T>((C2<T> *)d_ptw_fine_, (C2<T> *)d_ptw_coarse_, shift, cn_, s),
                   "post twiddle generation");
     }
     check_hip(hipStreamSynchronize(s), "plan table sync");
@@ -584,8 +584,8 @@ PassArgs<T> Plan::pass_args(size_t p, int inverse) const {
     a.post_tw_shift = ptw_shift_;
     a.real_tw = (const C2<T> *)d_real_tw_;
     a.rot = (const C2<T> *)d_rot_;
-    a.ptw_fine = d_ptw_fine_;
-    a.ptw_coarse = d_ptw_coarse_;
+    a.ptw_fine = (const C2<T> *)d_ptw_fine_;
+    a.ptw_coarse = (const C2<T> *)d_ptw_coarse_;
     a.in_from_out = pp.in_from_out;
     return a;
 }

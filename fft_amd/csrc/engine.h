@@ -132,8 +132,8 @@ class Plan {
     size_t tw_count_ = 0;
     void *d_real_tw_ = nullptr;
     void *d_rot_ = nullptr;
-    C2<double> *d_ptw_fine_ = nullptr;
-    C2<double> *d_ptw_coarse_ = nullptr;
+    void *d_ptw_fine_ = nullptr;  // C2<T>, the plan's arithmetic type
+    void *d_ptw_coarse_ = nullptr;
     int ptw_shift_ = 0;
     size_t table_bytes_ = 0;
     std::unique_ptr<Plan> bs_inner_;

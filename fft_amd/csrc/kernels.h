@@ -52,9 +52,11 @@ template <typename T>
 hipError_t gen_real_twiddles(C2<T> *tmi, C2<T> *rot, size_t n, int half_shift,
                              hipStream_t stream);
 
-// Two-level FP64 table for W_N^m: fine[m] for m < 2^shift,
-// coarse[q] = W_N^{q 2^shift} for q < ceil(N / 2^shift).
-hipError_t gen_post_twiddles(C2<double> *fine, C2<double> *coarse, int shift, size_t N,
+// Two-level table for W_N^m: fine[m] for m < 2^shift,
+// coarse[q] = W_N^{q 2^shift} for q < ceil(N / 2^shift); computed in FP64,
+// rounded once to T.
+template <typename T>
+hipError_t gen_post_twiddles(C2<T> *fine, C2<T> *coarse, int shift, size_t N,
                              hipStream_t stream);
 
 // Bluestein (chirp-z) helpers, bluestein.hip.  w: chirp of length N;

@@ -137,23 +137,34 @@ template hipError_t gen_real_twiddles<float>(C2<float> *, C2<float> *, size_t, i
 template hipError_t gen_real_twiddles<double>(C2<double> *, C2<double> *, size_t, int,
                                               hipStream_t);
 
-__global__ void k_post_tw(C2<double> *fine, C2<double> *coarse, int shift, uint64_t N) {
+template <typename T>
+__global__ void k_post_tw(C2<T> *fine, C2<T> *coarse, int shift, uint64_t N) {
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t nf = 1ull << shift;
     const uint64_t nc = (N + nf - 1) >> shift;
-    if (i < nf) fine[i] = unit_root(i, N);
-    if (i < nc) coarse[i] = unit_root(i << shift, N);
+    if (i < nf) {
+        const C2<double> w = unit_root(i, N);
+        fine[i] = {(T)w.x, (T)w.y};
+    }
+    if (i < nc) {
+        const C2<double> w = unit_root(i << shift, N);
+        coarse[i] = {(T)w.x, (T)w.y};
+    }
 }
 
-hipError_t gen_post_twiddles(C2<double> *fine, C2<double> *coarse, int shift, size_t N,
+template <typename T>
+hipError_t gen_post_twiddles(C2<T> *fine, C2<T> *coarse, int shift, size_t N,
                              hipStream_t stream) {
     const size_t nf = 1ull << shift;
     const size_t nc = (N + nf - 1) >> shift;
     const size_t cnt = nf > nc ? nf : nc;
-    hipLaunchKernelGGL(k_post_tw, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, fine,
-                       coarse, shift, (uint64_t)N);
+    hipLaunchKernelGGL((k_post_tw<T>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream,
+                       fine, coarse, shift, (uint64_t)N);
     return hipGetLastError();
 }
+template hipError_t gen_post_twiddles<float>(C2<float> *, C2<float> *, int, size_t, hipStream_t);
+template hipError_t gen_post_twiddles<double>(C2<double> *, C2<double> *, int, size_t,
+                                              hipStream_t);
 
 // -------------------------------------------------------- synthetic inputs --
 

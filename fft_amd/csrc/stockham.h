@@ -67,8 +67,8 @@ struct PassArgs {
     int32_t post_tw_shift; // fine table covers 2^shift entries
     const C2<T> *real_tw;  // twiddlesMinusI: -i e^{-2 pi i (i [+1/2]) / N}, N/4 + 1 entries
     const C2<T> *rot;      // half-shift pre-rotations e^{-2 pi i i / N}, N/2 entries
-    const C2<double> *ptw_fine;    // W_N^{m}, m < 2^shift
-    const C2<double> *ptw_coarse;  // W_N^{m 2^shift}
+    const C2<T> *ptw_fine;    // W_N^{m}, m < 2^shift (FP64 values rounded once to T)
+    const C2<T> *ptw_coarse;  // W_N^{m 2^shift}
     int32_t in_from_out;   // second pass of a two-pass plan: reads the output buffer
     int32_t real_h;        // complex length h of a two-pass real transform
 };
@@ -124,7 +124,7 @@ struct Block {
     int conj_in_, conj_out_, mode_, half_shift_, post_tw_, post_tw_shift_, real_h_, mfma_;
     IoDesc iin_, iout_;
     const C2<T> *tw_, *real_tw_, *rot_;
-    const C2<double> *ptw_fine_, *ptw_coarse_;
+    const C2<T> *ptw_fine_, *ptw_coarse_;
 
     __device__ Block(const PassArgs<T> &args, const C2<T> *in_, C2<T> *out_, int total, C2<T> *l,
                      int t, int s0)
@@ -181,13 +181,12 @@ struct Block {
     }
 
     __device__ __forceinline__ C2<T> post_twiddle(int g, int e) const {
-        // W_N^{c e}, c = column index of slot g, via a two-level FP64 table.
+        // W_N^{c e}, c = column index of slot g, as the product of two table
+        // entries (fine x coarse), each an FP64 value rounded once to T.
         const uint32_t c = (uint32_t)g - fdiv((uint32_t)g, iout_.per_batch) * iout_.per_batch.d;
         // c < N2 and e < N1, so c e < N: no reduction needed.
         const uint32_t m = c * (uint32_t)e;
-        const C2<double> f = ptw_fine_[m & ((1u << post_tw_shift_) - 1u)];
-        const C2<double> h = ptw_coarse_[m >> post_tw_shift_];
-        return {(T)fmadd(f.x, h.x, -(f.y * h.y)), (T)fmadd(f.x, h.y, f.y * h.x)};
+        return cmul(ptw_fine_[m & ((1u << post_tw_shift_) - 1u)], ptw_coarse_[m >> post_tw_shift_]);
     }
 
     // One Stockham stage of radix R.
