@@ -227,7 +227,8 @@ class Server {
         if (r->ev) (void)hipEventSynchronize(r->ev);
     }
 
-    // enqueue the device work of an admitted request (mu_ held)
+    // enqueue the device work of an admitted request (worker thread, mu_ not
+    // held: uses only the streams, the event pool and the request)
     void launch(Req *r) {
         hipStream_t s = streams_[next_stream_++ % streams_.size()];
         char *in = pool_->base() + r->in_off;
@@ -282,13 +283,25 @@ class Server {
                 r->in_off = a;
                 r->out_off = b;
                 r->t_admit = now();
+                // The device work is enqueued (and, in plain mode, waited
+                // for) without the lock: launch() touches only this thread's
+                // state, so submit / poll / release are not held up.
+                int status = SFFT_OK;
+                lock.unlock();
                 try {
                     launch(r);
-                    inflight_.push_back(r);
                 } catch (const Error &e) {
+                    status = e.code;
+                } catch (const std::exception &) {
+                    status = SFFT_ERR_DEVICE;
+                }
+                lock.lock();
+                if (status == SFFT_OK) {
+                    inflight_.push_back(r);
+                } else {
                     pool_->release(a, r->in_bytes);
                     pool_->release(b, r->out_bytes);
-                    r->status = e.code;
+                    r->status = status;
                     r->t_done = now();
                     complete(r);
                 }
