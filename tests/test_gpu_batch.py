@@ -113,3 +113,36 @@ def test_mixed_kinds_in_one_launch():
         if max(errs) > TOL32 or not same:
             bad.append((n, k, d, errs, same))
     assert not bad, bad
+
+
+def test_uneven_batch_double():
+    sizes, dirs = workloads.uneven_batch(seed=9, count=300)
+    kinds = [L.C2C if i % 3 else (L.R2C if i % 2 else L.R2C_HALFSHIFT) for i in range(300)]
+    sizes = [int(s) + (int(s) % 2 if k != L.C2C else 0) for s, k in zip(sizes, kinds)]
+    sizes = [s if fft_amd.Plan(s, k, L.F64).info()["chirp_length"] == 0 else 2 * 1024
+             for s, k in zip(sizes, kinds)]
+    b = fft_amd.UnevenBatch(sizes, [int(d) for d in dirs], kinds, precision=L.F64)
+    xin = torch.empty(b.in_bytes, dtype=torch.uint8, device="cuda")
+    fft_amd.fill(xin.view(torch.float64), seed=9, dist="uniform")
+    out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    ref = torch.empty_like(out)
+    b.execute(xin, out)
+    b.execute_per_size(xin, ref)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    for i in range(0, 300, 29):
+        n, k, d = sizes[i], kinds[i], int(dirs[i])
+        hs = k == L.R2C_HALFSHIFT
+        nb = 16 * n if k == L.C2C else 8 * n
+        ib = xin[b.in_offsets[i]:b.in_offsets[i] + nb]
+        ob = out[b.out_offsets[i]:b.out_offsets[i] + nb]
+        if k == L.C2C:
+            want = po.fft(ib.view(torch.complex128).cpu().numpy(), inverse=bool(d))
+            got = ob.view(torch.complex128).cpu().numpy()
+        elif d == 0:
+            want = po.real_fft(ib.view(torch.float64).cpu().numpy(), half_shift=hs)
+            got = ob.view(torch.complex128).cpu().numpy()
+        else:
+            want = po.real_ifft(ib.view(torch.complex128).cpu().numpy(), half_shift=hs)
+            got = ob.view(torch.float64).cpu().numpy()
+        assert po.rel_l2(got, want) <= 1e-12, (n, k, d)

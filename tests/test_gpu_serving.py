@@ -90,3 +90,22 @@ def test_poisson_run_small():
     s = latency_summary(rec)
     assert s["requests"] > 300 and s["failed"] == 0, s
     assert s["p50_ms"] > 0 and s["p99_ms"] >= s["p50_ms"]
+
+
+def test_double_precision_request_matches_plain_plan():
+    """An F64 server: a retained request equals the same transform run alone."""
+    n, batch, seed = 6000, 3, 11
+    with Server(budget_bytes=1 << 30, precision=L.F64) as srv:
+        srv.submit(1, n, L.C2C, L.INVERSE, batch, seed, retain=True)
+        srv.drain()
+        got = srv.fetch(1, batch * n * 16).view(np.complex128).reshape(batch, n)
+        srv.release(1)
+    plan = fft_amd.Plan(n, L.C2C, L.F64)
+    x = torch.empty(batch, n, dtype=torch.complex128, device="cuda")
+    y = torch.empty_like(x)
+    fft_amd.fill(x, seed=seed)
+    plan.execute_ptr(x.data_ptr(), y.data_ptr(), batch, L.INVERSE,
+                     torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(got, y.cpu().numpy())
+    assert po.rel_l2(got, po.fft(x.cpu().numpy(), inverse=True)) <= 1e-12
