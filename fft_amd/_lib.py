@@ -65,6 +65,7 @@ class PlanInfo(ctypes.Structure):
         ("table_bytes", ctypes.c_size_t),
         ("mfma_stages", ctypes.c_int),
         ("chirp_length", ctypes.c_size_t),
+        ("in_place", ctypes.c_int),
     ]
 
 

@@ -83,6 +83,7 @@ class Plan:
             "transforms_per_block": inf.transforms_per_block,
             "workspace_bytes": inf.workspace_bytes, "table_bytes": inf.table_bytes,
             "mfma_stages": inf.mfma_stages, "chirp_length": inf.chirp_length,
+            "in_place": inf.in_place,
         }
 
     def execute_ptr(self, in_ptr: int, out_ptr: int, batch: int, direction: int,

@@ -199,6 +199,7 @@ int sfft_plan_get_info(sfft_plan plan, sfft_plan_info *info) {
         info->workspace_bytes = 0;
         info->table_bytes = p.table_bytes();
         info->mfma_stages = p.mfma_stages();
+        info->in_place = p.in_place() ? 1 : 0;
         return SFFT_OK;
     });
 }

@@ -692,8 +692,16 @@ void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, s
     const size_t in_len = ((batch - 1) * in_dist + default_in_dist(inverse)) * in_unit;
     const size_t out_len = ((batch - 1) * out_dist + default_out_dist(inverse)) * out_unit;
     const uintptr_t i0 = (uintptr_t)in, o0 = (uintptr_t)out;
-    if (i0 < o0 + out_len && o0 < i0 + in_len)
-        throw Error(SFFT_ERR_ALIASING, "input and output overlap (transforms are out of place)");
+    if (i0 == o0 && in_dist * in_unit == out_dist * out_unit) {
+        // in place, same layout: every transform is read whole before it is
+        // written (single pass: one workgroup per slot, reads before the
+        // first barrier; chirp-z: the input is consumed into the workspace)
+        if (!in_place())
+            throw Error(SFFT_ERR_ALIASING,
+                        "in-place needs a one-pass or chirp-z plan (this length runs two passes)");
+    } else if (i0 < o0 + out_len && o0 < i0 + in_len) {
+        throw Error(SFFT_ERR_ALIASING, "input and output overlap other than exactly in place");
+    }
     if (in_dist < default_in_dist(inverse) || out_dist < default_out_dist(inverse))
         throw Error(SFFT_ERR_INVALID_ARG, "batch distance shorter than one transform");
     if (prec_ == kF64)

@@ -85,6 +85,8 @@ class Plan {
     const std::vector<PassPlan> &passes() const { return passes_; }
     size_t table_bytes() const { return table_bytes_; }
     int mfma_stages() const;
+    // Whether in == out (same layout) is allowed: one-pass and chirp-z plans.
+    bool in_place() const { return bs_inner_ != nullptr || passes_.size() == 1; }
     // Chirp-z plan: the length has a prime factor the kernels do not plan;
     // the transform runs as a convolution through FFTs of length bluestein_m().
     bool bluestein() const { return bs_inner_ != nullptr; }

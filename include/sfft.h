@@ -11,8 +11,11 @@
  *  - A batch is `batch` transforms stored back to back (stride n complex
  *    values for C2C, n reals / n/2 complex values for the real kinds).
  *  - All data pointers passed to sfft_execute* are DEVICE pointers (HBM).
- *  - Transforms are out of place; `in` is never written.  in == out is
- *    rejected with SFFT_ERR_ALIASING (the reference silently corrupts).
+ *  - Transforms are out of place, or exactly in place (in == out with the
+ *    same byte layout) for plans whose sfft_plan_info.in_place is 1 (one-pass
+ *    and chirp-z plans); any other overlap of input and output, and in place
+ *    with a two-pass plan, returns SFFT_ERR_ALIASING (the reference silently
+ *    corrupts in place).  An out-of-place call never writes `in`.
  *  - The inverse is unnormalised: ifft(fft(x)) = n * x (reference
  *    signalsmith-fft.h:295-315, no 1/N scaling).
  *  - Streams are hipStream_t passed as void*; NULL is the default stream.
@@ -79,6 +82,7 @@ typedef struct {
     int mfma_stages;        /* radix-16 stages run as MFMA contractions (16-bit storage) */
     size_t chirp_length;    /* lengths with prime factors > 13: the plan is a chirp-z
                                convolution through FFTs of this length (0: direct) */
+    int in_place;           /* 1: in == out (same layout) is allowed */
 } sfft_plan_info;
 
 /* ---- library ---- */

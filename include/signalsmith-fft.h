@@ -19,7 +19,8 @@
 // throwing signalsmith::FFTError (the reference has no error path).
 //
 // Differences the reference leaves undefined, defined here: in-place use
-// (input == output) is supported by the host path (staging separates them);
+// (input == output) is supported by the host path (staging separates them)
+// and, on device pointers, by one-pass and chirp-z lengths (others throw);
 // odd RealFFT sizes round down to even.  Link with -lsfft.
 #ifndef SIGNALSMITH_FFT_GPU_H
 #define SIGNALSMITH_FFT_GPU_H

@@ -233,12 +233,19 @@ def test_two_pass_double():
     assert po.rel_l2(_run(n, x, dtype=torch.complex128), po.fft(x)) <= 1e-12
 
 
-def test_aliasing_rejected():
+def test_aliasing_rejected_in_place_allowed():
+    # partial overlap: rejected; exactly in place on a one-pass length: the
+    # same values as out of place (the reference corrupts in place)
     f = fft_amd.FFT(64)
-    x = torch.zeros(4, 64, dtype=torch.complex64, device="cuda")
+    x = torch.randn(4, 64, dtype=torch.complex64, device="cuda")
     with pytest.raises(L.SfftError) as e:
-        f.fft(x, x)
+        f.fft(x[:3], x[1:])
     assert e.value.code == L.SFFT_ERR_ALIASING
+    want = torch.empty_like(x)
+    f.fft(x, want)
+    f.fft(x, x)
+    torch.cuda.synchronize()
+    assert torch.equal(x, want)
 
 
 def test_setters_mirror_reference():

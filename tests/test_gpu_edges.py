@@ -154,3 +154,35 @@ def test_largest_lengths_single_bin(n, prec):
     assert rel < (1e-6 if prec == L.F32 else 1e-12), rel
     del x, y
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n,kind", [(4096, L.C2C), (777, L.C2C), (17, L.C2C), (2048, L.R2C),
+                                    (6000, L.R2C_HALFSHIFT), (34, L.R2C)])
+def test_in_place_one_pass_and_chirp_z(n, kind):
+    """in == out with the same layout: results equal the out-of-place ones."""
+    p = fft_amd.Plan(n, kind, L.F32)
+    assert p.info()["in_place"] == 1
+    batch = 6
+    sh = torch.cuda.current_stream().cuda_stream
+    for d in (0, 1):
+        if kind == L.C2C or d == 1:
+            x = torch.randn(batch, n if kind == L.C2C else n // 2, dtype=torch.complex64,
+                            device="cuda")
+        else:
+            x = torch.randn(batch, n, dtype=torch.float32, device="cuda")
+        nbytes = x.numel() * x.element_size()
+        want = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        p.execute_ptr(x.data_ptr(), want.data_ptr(), batch, d, sh)
+        buf = x.clone()
+        p.execute_ptr(buf.data_ptr(), buf.data_ptr(), batch, d, sh)
+        torch.cuda.synchronize()
+        assert torch.equal(buf.view(torch.uint8).flatten(), want), (n, kind, d)
+
+
+def test_in_place_two_pass_rejected():
+    p = fft_amd.Plan(1 << 16, L.C2C, L.F32)
+    assert p.info()["in_place"] == 0
+    x = torch.zeros(2, 1 << 16, dtype=torch.complex64, device="cuda")
+    rc = L.lib().sfft_execute(p._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                              2, 0, None)
+    assert rc == L.SFFT_ERR_ALIASING
