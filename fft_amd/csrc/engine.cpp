@@ -261,6 +261,31 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     return true;
 }
 
+// A compiled column kernel for a pass of column length n (first pass
+// kModeColRow, second kModeColCol): the pass takes its radices, threads per
+// column and columns per block, so the runtime-planned kernel (used by uneven
+// batches and strided calls) computes the same bits.
+static void use_fixed_columns(PassPlan &pp, size_t n, int prec, int mode) {
+    const FixedKernel *fk = find_fixed((int)n, prec, 2, -1);
+    if (!fk) return;
+    std::vector<int> rs(fk->radices, fk->radices + fk->nst);
+    int ept = 0;
+    for (int e : {kEptSmall, kEptLarge})
+        if (!ept && threads_for((int)n, rs, e) <= fk->tpt) ept = e;
+    if (!ept || fk->tpt * fk->tpw > max_block(prec)) return;
+    const size_t slot_lds = pp.lds_bytes / (size_t)pp.pd.tpw;
+    PassDesc pd = pp.pd;
+    fill_stages(pd, (int)n, rs);
+    pd.tpt = fk->tpt;
+    pd.tpw = fk->tpw;
+    pp.pd = pd;
+    pp.ept = ept;
+    pp.block = (unsigned)(fk->tpt * fk->tpw);
+    pp.lds_bytes = slot_lds * (size_t)fk->tpw;
+    pp.fixed = fk;
+    pp.fixed_grid[mode] = (unsigned)(fk->blocks_per_cu(mode) * device_info().cu_count);
+}
+
 // 16-bit storage: run the pass' radix-16 stages on MFMA when every wave of
 // the block takes whole groups of 32 butterflies (stockham.h stage16_mfma).
 static void enable_mfma(PassPlan &pp) {
@@ -390,6 +415,12 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         p2.out_elem_stride = (int64_t)n1;
         p2.pd.in_col = 1;
         p2.pd.out_col = 1;
+        // compiled column kernels (complex plans without layout hints): the
+        // pass takes the kernel's radices and tile
+        if (kind == kC2C && !(h && (h->tpw || h->fixed_variant == -2))) {
+            use_fixed_columns(p1, n1, prec, kModeColRow);
+            use_fixed_columns(p2, n2, prec, kModeColCol);
+        }
         passes_.push_back(p1);
         passes_.push_back(p2);
     }
@@ -600,7 +631,7 @@ size_t Plan::slot_lds_bytes(size_t p) const {
 template <typename T>
 void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
                int inverse, hipStream_t stream) const {
-    if (!passes_.empty() && passes_[0].fixed && in_dist == default_in_dist(inverse) && out_dist == default_out_dist(inverse)) {
+    if (passes_.size() == 1 && passes_[0].fixed && in_dist == default_in_dist(inverse) && out_dist == default_out_dist(inverse)) {
         run_fixed<T>(in, out, batch, inverse, stream);
         return;
     }
@@ -624,6 +655,10 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         const PassPlan &pp = passes_[p];
         const size_t slots = batch * (size_t)pp.slots_per_transform;
         if (slots > 0x7fffffffull) throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
+        if (pp.fixed && pp.fixed->real == 2) {  // compiled column kernel
+            run_fixed_columns<T>(p, in, din, out, dout, batch, inverse, stream);
+            continue;
+        }
         PassArgs<T> a = pass_args<T>(p, inverse);
         a.iin.batch_stride = pp.in_from_out ? dout : din;
         a.iout.batch_stride = dout;
@@ -679,6 +714,55 @@ void Plan::run_fixed(const void *in, void *out, size_t batch, int inverse,
     }
 }
 template void Plan::run_fixed<float>(const void *, void *, size_t, int, hipStream_t) const;
+
+// Pass p of a two-pass complex plan through its compiled column kernel.  The
+// kernel addresses HBM with 32-bit byte offsets from the launch base, so the
+// batch is split into launches spanning less than 4 GiB.
+template <typename T>
+void Plan::run_fixed_columns(size_t p, const void *in, int64_t din, void *out, int64_t dout,
+                             size_t batch, int inverse, hipStream_t stream) const {
+    const PassPlan &pp = passes_[p];
+    const FixedKernel &fk = *pp.fixed;
+    const int mode = p == 0 ? kModeColRow : kModeColCol;
+    const size_t eb = sizeof(C2<T>);
+    const int64_t dmax = std::max(pp.in_from_out ? dout : din, dout);
+    const size_t chunk = std::max<size_t>(1, ((1ull << 32) - 1) / ((size_t)dmax * eb));
+    if ((size_t)pp.slots_per_transform * std::min(chunk, batch) > 0x7fffffffull)
+        throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
+    const uint32_t flip = 0x80000000u;
+    // first stage with twiddles (stage 1): the pass' part of the dense table
+    const int32_t tw0 = pp.pd.nstages > 1 ? pp.pd.st[1].tw_off : 0;
+    for (size_t c0 = 0; c0 < batch; c0 += chunk) {
+        const size_t cb = std::min(chunk, batch - c0);
+        FixedArgs<T> a{};
+        const C2<T> *src = pp.in_from_out ? (const C2<T> *)out : (const C2<T> *)in;
+        const int64_t ds = pp.in_from_out ? dout : din;
+        a.in = src + (int64_t)c0 * ds;
+        a.out = (C2<T> *)out + (int64_t)c0 * dout;
+        a.tw = (const C2<T> *)d_tw_ + tw0;
+        a.total_slots = (int32_t)(cb * (size_t)pp.slots_per_transform);
+        a.flip_in = (inverse && p == 0) ? flip : 0u;
+        a.flip_out = (inverse && p + 1 == passes_.size()) ? flip : 0u;
+        a.spt = make_fastdiv((uint32_t)pp.slots_per_transform);
+        a.in_bs = (int32_t)ds;
+        a.in_ss = (int32_t)pp.in_slot_stride;
+        a.in_es = (int32_t)pp.in_elem_stride;
+        a.out_bs = (int32_t)dout;
+        a.out_ss = (int32_t)pp.out_slot_stride;
+        a.out_es = (int32_t)pp.out_elem_stride;
+        a.post_tw = pp.post_tw;
+        a.ptw_shift = ptw_shift_;
+        a.ptw_fine = (const C2<T> *)d_ptw_fine_;
+        a.ptw_coarse = (const C2<T> *)d_ptw_coarse_;
+        const size_t groups = ((size_t)a.total_slots + fk.tpw - 1) / fk.tpw;
+        const unsigned grid = (unsigned)std::min<size_t>(groups, pp.fixed_grid[mode]);
+        check_hip(fk.launch(&a, mode, grid, stream), "column-pass launch");
+    }
+}
+template void Plan::run_fixed_columns<float>(size_t, const void *, int64_t, void *, int64_t,
+                                             size_t, int, hipStream_t) const;
+template void Plan::run_fixed_columns<double>(size_t, const void *, int64_t, void *, int64_t,
+                                              size_t, int, hipStream_t) const;
 template void Plan::run_fixed<double>(const void *, void *, size_t, int, hipStream_t) const;
 
 void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,

@@ -58,7 +58,7 @@ struct PassPlan {
     size_t lds_bytes = 0;
     // compile-time planned kernel with the same radices (contiguous batches)
     const FixedKernel *fixed = nullptr;
-    unsigned fixed_grid[3] = {0, 0, 0};  // resident blocks per mode (C2C, R2C, C2R)
+    unsigned fixed_grid[8] = {};  // resident blocks per mode (kModeC2C ... kModeColCol)
 };
 
 // Overrides the planner's choice (autotuner candidates).
@@ -119,6 +119,9 @@ class Plan {
              hipStream_t stream) const;
     template <typename T>
     void run_fixed(const void *in, void *out, size_t batch, int inverse, hipStream_t stream) const;
+    template <typename T>
+    void run_fixed_columns(size_t p, const void *in, int64_t din, void *out, int64_t dout,
+                           size_t batch, int inverse, hipStream_t stream) const;
     template <typename T>
     void build_bluestein();
     template <typename T>

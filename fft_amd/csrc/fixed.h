@@ -31,6 +31,14 @@ struct FixedArgs {
     uint32_t flip_in;      // sign-bit mask xor-ed into Im at the HBM load (inverse)
     uint32_t flip_out;     // ... and at the HBM store
     int32_t half_shift;
+    // column passes of two-pass plans: slot g is column c = g mod spt of
+    // transform b = g / spt; element e of it sits at b bs + c ss + e es
+    // (element strides of the input and of the output)
+    FastDiv spt;
+    int32_t in_bs, in_ss, in_es;
+    int32_t out_bs, out_ss, out_es;
+    int32_t post_tw, ptw_shift;  // first pass: output element e times W_N^{c e}
+    const C2<T> *ptw_fine, *ptw_coarse;
 };
 
 // Sign flip of the imaginary part by a uniform xor mask (0 or the sign bit).
@@ -67,6 +75,11 @@ struct Fixed {
     static constexpr int NST = sizeof...(Rs);
     static constexpr int RAD[NST] = {Rs...};
     static constexpr int THREADS = TPT * TPW;
+    // Column modes: stage 0 (and the last stage of the second pass) map
+    // adjacent lanes to adjacent slots (columns), so the strided HBM accesses
+    // coalesce across columns; the other stages keep slot-major lanes.
+    static constexpr bool COL = MODE == kModeColRow || MODE == kModeColCol;
+    static constexpr bool COL_OUT = MODE == kModeColCol;
     static constexpr int STRIDE = N + (N >> kLdsPadShift) + 1;  // lds_padded(N)
     static constexpr uint32_t EB = sizeof(C2<T>);
     static constexpr size_t LDS_BYTES =
@@ -92,6 +105,19 @@ struct Fixed {
     static constexpr int PF_STAGE =
         PFS >= 0 ? (PFS < NST ? PFS : NST - 1) : ((NST + PFS) > 0 ? (NST + PFS) : 0);
     static_assert(ns_of(NST) == N, "radices must multiply to N");
+    static_assert(!COL || NST >= 2, "column kernels need two stages or more");
+
+    // thread -> (slot, element lane) of stage S; gbase = first slot of the group
+    template <int S>
+    static __device__ __forceinline__ void map_of(const FixedArgs<T> &a, int gbase, int &g,
+                                                  int &slot, int &lid, bool &active) {
+        constexpr bool CM = COL && (S == 0 || (S == NST - 1 && COL_OUT));
+        const int tid = (int)threadIdx.x;
+        slot = CM ? tid % TPW : tid / TPT;
+        lid = CM ? tid / TPW : tid % TPT;
+        g = gbase + slot;
+        active = g < a.total_slots;
+    }
 
     // twiddles of stage S for this thread's butterflies
     template <int S, bool = (S < NST)>
@@ -135,12 +161,29 @@ struct Fixed {
         }
     }
 
-    // stage-0 element loads of slot g into pf (ROW mapping, contiguous batch)
+    // stage-0 element loads of slot g into pf (contiguous batch, or a column)
     static __device__ __forceinline__ void load0(const FixedArgs<T> &a, C2<T> (&pf)[PF], int g,
                                                  int lid) {
         constexpr int R = RAD[0], NB = N / R, B = bfly_of(0);
         if (g >= a.total_slots) return;
         const char *src = reinterpret_cast<const char *>(a.in);
+        if constexpr (COL) {
+            const uint32_t t = fdiv((uint32_t)g, a.spt);
+            const uint32_t c = (uint32_t)g - t * a.spt.d;
+            const uint32_t es = (uint32_t)a.in_es * EB;
+            const uint32_t lane =
+                (t * (uint32_t)a.in_bs + c * (uint32_t)a.in_ss) * EB + (uint32_t)lid * es;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                if ((NB % TPT) == 0 || lid + b * TPT < NB) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        pf[b * R + r] = *reinterpret_cast<const C2<T> *>(
+                            src + (lane + (uint32_t)(b * TPT + r * NB) * es));
+                }
+            }
+            return;
+        }
         const uint32_t lane = ((uint32_t)g * (uint32_t)N + (uint32_t)lid) * EB;
 #pragma unroll
         for (int b = 0; b < B; ++b) {
@@ -157,10 +200,22 @@ struct Fixed {
     // PF_HERE: also load stage S+1's twiddles into tn, then issue the next
     // group's first-stage loads into pf.
     template <int S, bool TW_READY, bool PF_HERE>
-    static __device__ __forceinline__ void stage(const FixedArgs<T> &a, char *lds_c, int g,
-                                                 bool active, int slot, int lid, C2<T> (&pf)[PF],
-                                                 int g_next, TwBuf<S> &tb, TwBuf<S + 1> &tn) {
+    static __device__ __forceinline__ void stage(const FixedArgs<T> &a, char *lds_c, int g_in,
+                                                 bool active_in, int slot_in, int lid_in,
+                                                 C2<T> (&pf)[PF], int g_next, TwBuf<S> &tb,
+                                                 TwBuf<S + 1> &tn) {
         constexpr int R = RAD[S], NS = ns_of(S), NB = N / R, B = bfly_of(S);
+        // column modes: g_in is the group's first slot and each stage maps
+        // its own lanes (the next stage's twiddles and the prefetch use theirs)
+        int g = g_in, slot = slot_in, lid = lid_in, lid_n = lid_in, lid_pf = lid_in;
+        bool active = active_in, active_n = active_in;
+        if constexpr (COL) {
+            map_of<S>(a, g_in, g, slot, lid, active);
+            int gx, sx;
+            map_of<S + 1>(a, g_in, gx, sx, lid_n, active_n);
+            bool ax;
+            map_of<0>(a, g_in, gx, sx, lid_pf, ax);
+        }
         constexpr bool FIRST = (S == 0), LAST = (S == NST - 1);
         constexpr bool SRC_PF = FIRST && MODE != kModeC2R;
         constexpr bool DST_G = LAST && MODE != kModeR2C;
@@ -183,11 +238,11 @@ struct Fixed {
         if constexpr (!TW_READY) load_tw<S>(a, tb, active, lid);
         if constexpr (PF_HERE) {
             if constexpr (SCH::PF_FIRST) {
-                if (MODE != kModeC2R && g_next >= 0) load0(a, pf, g_next, lid);
-                load_tw<S + 1>(a, tn, active, lid);
+                if (MODE != kModeC2R && g_next >= 0) load0(a, pf, g_next, lid_pf);
+                load_tw<S + 1>(a, tn, active_n, lid_n);
             } else {
-                load_tw<S + 1>(a, tn, active, lid);
-                if (MODE != kModeC2R && g_next >= 0) load0(a, pf, g_next, lid);
+                load_tw<S + 1>(a, tn, active_n, lid_n);
+                if (MODE != kModeC2R && g_next >= 0) load0(a, pf, g_next, lid_pf);
             }
         }
         if constexpr (SRC_PF) {
@@ -232,7 +287,32 @@ struct Fixed {
                 dft<R, T>(v[b]);
             }
         }
-        if constexpr (DST_G) {
+        if constexpr (DST_G && COL) {
+            char *dst = reinterpret_cast<char *>(a.out);
+            const uint32_t t = fdiv((uint32_t)g, a.spt);
+            const uint32_t c = (uint32_t)g - t * a.spt.d;
+            const uint32_t es = (uint32_t)a.out_es * EB;
+            const uint32_t sbase = (t * (uint32_t)a.out_bs + c * (uint32_t)a.out_ss) * EB;
+            const uint32_t mask = (1u << a.ptw_shift) - 1u;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lid + b * TPT;
+                if (active && jvalid(b)) {
+                    const int o = (j / NS) * NS * R + j % NS;
+#pragma unroll
+                    for (int t2 = 0; t2 < R; ++t2) {
+                        const uint32_t e = (uint32_t)(o + t2 * NS);
+                        C2<T> y = v[b][t2];
+                        flip_im(y, a.flip_out);
+                        if (a.post_tw) {
+                            const uint32_t m = c * e;
+                            y = cmul(y, cmul(a.ptw_fine[m & mask], a.ptw_coarse[m >> a.ptw_shift]));
+                        }
+                        *reinterpret_cast<C2<T> *>(dst + (sbase + e * es)) = y;
+                    }
+                }
+            }
+        } else if constexpr (DST_G) {
             char *dst = reinterpret_cast<char *>(a.out);
             const uint32_t sbase = (uint32_t)g * (uint32_t)N;
 #pragma unroll
@@ -347,15 +427,38 @@ struct Fixed {
     static __device__ __forceinline__ void run(const FixedArgs<T> &a, char *lds_c) {
         const int tid = (int)threadIdx.x;
         const int slot = tid / TPT, lid = tid % TPT;
+        // stage-0 lanes (the prefetch): column-major in the column modes
+        const int slot0 = COL ? tid % TPW : slot, lid0 = COL ? tid / TPW : lid;
         const int ngroups = (a.total_slots + TPW - 1) / TPW;
         C2<T> pf[PF];
+        if constexpr (COL) {
+            // Column tiles narrower than 128 bytes share HBM lines with their
+            // neighbours: then each window of gridDim.x groups is numbered so
+            // that every XCD (blocks go round-robin over them) runs a
+            // contiguous run of groups, whose shared lines meet in one L2.
+            const int G = (int)gridDim.x;
+            constexpr bool NARROW = TPW * (int)EB < 128;
+            const int off = NARROW ? (int)xcd_tile(blockIdx.x, (unsigned)G) : (int)blockIdx.x;
+            int grp = off;
+            if (grp < ngroups) load0(a, pf, grp * TPW + slot0, lid0);
+            for (int w = 0; w < ngroups; w += G, grp += G) {
+                if (grp >= ngroups) break;  // the last window's spare blocks
+                const int gn = grp + G;
+                const int g_next = gn < ngroups ? gn * TPW + slot0 : -1;
+                TwBuf<0> t0;
+                stages_from<0, false>(a, lds_c, grp * TPW, grp * TPW < a.total_slots, slot, lid,
+                                      pf, g_next, t0);
+            }
+            return;
+        }
         int grp = (int)blockIdx.x;
-        if (MODE != kModeC2R && grp < ngroups) load0(a, pf, grp * TPW + slot, lid);
+        if (MODE != kModeC2R && grp < ngroups) load0(a, pf, grp * TPW + slot0, lid0);
         for (; grp < ngroups; grp += (int)gridDim.x) {
-            const int g = grp * TPW + slot;
+            // column modes pass the group's first slot; the stages map lanes
+            const int g = grp * TPW + (COL ? 0 : slot);
             const bool active = g < a.total_slots;
             const int gn = grp + (int)gridDim.x;
-            const int g_next = gn < ngroups ? gn * TPW + slot : -1;
+            const int g_next = gn < ngroups ? gn * TPW + slot0 : -1;
             if constexpr (MODE == kModeC2R) real_pre(a, lds_c, g, active, slot, lid);
             TwBuf<0> t0;
             stages_from<0, false>(a, lds_c, g, active, slot, lid, pf, g_next, t0);
@@ -376,7 +479,8 @@ struct FixedKernel {
     int n;          // complex length of the core transform
     int variant;    // schedule variant (0 = default)
     int prec;       // 0 F32, 1 F64
-    int real;       // 0: C2C kernel; 1: real pair (R2C forward + C2R inverse)
+    int real;       // 0: C2C kernel; 1: real pair (R2C forward + C2R inverse);
+                    // 2: column passes of two-pass plans (kModeColRow, kModeColCol)
     int tpt, tpw;
     int nst;
     int radices[kMaxStages];
@@ -472,6 +576,34 @@ struct FixedReal {
         };
         k.blocks_per_cu = [](int mode) {
             return mode == kModeC2R ? fixed_occupancy<FI>() : fixed_occupancy<FR>();
+        };
+        register_fixed(k);
+    }
+};
+
+// Registers the column-pass pair of one compile-time plan of column length
+// N (registry kind 2): the first pass of a two-pass plan (kModeColRow) and
+// the second (kModeColCol).
+template <typename T, int N, int TPT, int TPW, class SCH, int VARIANT, int... Rs>
+struct FixedCol {
+    using F1 = Fixed<T, N, TPT, TPW, kModeColRow, SCH, Rs...>;
+    using F2 = Fixed<T, N, TPT, TPW, kModeColCol, SCH, Rs...>;
+    FixedCol() {
+        FixedKernel k{};
+        k.n = N;
+        k.variant = VARIANT;
+        k.prec = sizeof(T) == 8 ? 1 : 0;
+        k.real = 2;
+        k.tpt = TPT;
+        k.tpw = TPW;
+        fixed_fill_radices<Rs...>(k);
+        k.lds_bytes = F1::LDS_BYTES;
+        k.launch = [](const void *args, int mode, unsigned grid, hipStream_t s) {
+            return mode == kModeColCol ? fixed_launch_one<F2>(args, grid, s)
+                                       : fixed_launch_one<F1>(args, grid, s);
+        };
+        k.blocks_per_cu = [](int mode) {
+            return mode == kModeColCol ? fixed_occupancy<F2>() : fixed_occupancy<F1>();
         };
         register_fixed(k);
     }

@@ -40,6 +40,9 @@ enum : int32_t {
     kModeR2CLoad = 3,   // first pass: half-shift rotation at the HBM load
     kModeC2RLoad = 4,   // first pass: pre-process from in[i], in[h-i] at the HBM load
     kModeC2RStore = 5,  // second pass: half-shift rotation at the HBM store
+    // compiled column-pass kernels of two-pass plans (fixed.h only)
+    kModeColRow = 6,  // first pass: column loads, four-step twiddle, row stores
+    kModeColCol = 7,  // second pass: column loads and column stores
 };
 
 // The pair loops of the real pre-/post-process (signalsmith-fft.h:463-472,
