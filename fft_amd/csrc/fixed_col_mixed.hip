@@ -1,0 +1,30 @@
+// fixed_col_mixed.hip -- compile-time column passes of two-pass plans for the
+// mixed-radix column lengths the planner picks most (by bytes) for complex
+// FP32 lengths between 19200 and 2^18; radices as the runtime planner's cost
+// model chooses them at 16 elements per thread.
+#include "fixed.h"
+
+namespace sfft {
+namespace {
+const FixedCol<float, 480, 32, 16, Sched<0, 0, 0, 0>, 0, 16, 15, 2> col480;
+const FixedCol<float, 375, 25, 16, Sched<0, 0, 0, 0>, 0, 25, 15> col375;
+const FixedCol<float, 486, 54, 8, Sched<0, 0, 0, 0>, 0, 9, 9, 6> col486;
+const FixedCol<float, 405, 45, 8, Sched<0, 0, 0, 0>, 0, 9, 9, 5> col405;
+const FixedCol<float, 500, 50, 8, Sched<0, 0, 0, 0>, 0, 10, 10, 5> col500;
+const FixedCol<float, 400, 25, 16, Sched<0, 0, 0, 0>, 0, 25, 16> col400;
+const FixedCol<float, 384, 26, 16, Sched<0, 0, 0, 0>, 0, 16, 8, 3> col384;
+const FixedCol<float, 432, 36, 8, Sched<0, 0, 0, 0>, 0, 12, 6, 6> col432;
+const FixedCol<float, 450, 30, 16, Sched<0, 0, 0, 0>, 0, 15, 15, 2> col450;
+const FixedCol<float, 324, 36, 8, Sched<0, 0, 0, 0>, 0, 9, 6, 6> col324;
+const FixedCol<float, 320, 22, 16, Sched<0, 0, 0, 0>, 0, 16, 5, 4> col320;
+const FixedCol<float, 250, 25, 16, Sched<0, 0, 0, 0>, 0, 25, 10> col250;
+const FixedCol<float, 243, 27, 16, Sched<0, 0, 0, 0>, 0, 9, 9, 3> col243;
+const FixedCol<float, 360, 24, 16, Sched<0, 0, 0, 0>, 0, 15, 8, 3> col360;
+const FixedCol<float, 300, 20, 16, Sched<0, 0, 0, 0>, 0, 15, 5, 4> col300;
+const FixedCol<float, 270, 23, 16, Sched<0, 0, 0, 0>, 0, 15, 6, 3> col270;
+const FixedCol<float, 288, 24, 16, Sched<0, 0, 0, 0>, 0, 12, 12, 2> col288;
+const FixedCol<float, 625, 25, 8, Sched<0, 0, 0, 0>, 0, 25, 25> col625;
+const FixedCol<float, 225, 15, 32, Sched<0, 0, 0, 0>, 0, 15, 15> col225;
+const FixedCol<float, 240, 16, 32, Sched<0, 0, 0, 0>, 0, 16, 15> col240;
+}  // namespace
+}  // namespace sfft
