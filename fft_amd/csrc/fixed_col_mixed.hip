@@ -1,6 +1,6 @@
 // fixed_col_mixed.hip -- compile-time column passes of two-pass plans for the
-// mixed-radix column lengths the planner picks most (by bytes) for complex
-// FP32 lengths between 19200 and 2^18; radices as the runtime planner's cost
+// mixed-radix column lengths the planner picks (by bytes) for complex FP32
+// lengths between 16384 and 2^18; radices as the runtime planner's cost
 // model chooses them at 16 elements per thread.
 #include "fixed.h"
 
@@ -26,5 +26,18 @@ const FixedCol<float, 288, 24, 16, Sched<0, 0, 0, 0>, 0, 12, 12, 2> col288;
 const FixedCol<float, 625, 25, 8, Sched<0, 0, 0, 0>, 0, 25, 25> col625;
 const FixedCol<float, 225, 15, 32, Sched<0, 0, 0, 0>, 0, 15, 15> col225;
 const FixedCol<float, 240, 16, 32, Sched<0, 0, 0, 0>, 0, 16, 15> col240;
+const FixedCol<float, 200, 13, 32, Sched<0, 0, 0, 0>, 0, 25, 8> col200;
+const FixedCol<float, 216, 18, 16, Sched<0, 0, 0, 0>, 0, 6, 6, 6> col216;
+const FixedCol<float, 192, 16, 32, Sched<0, 0, 0, 0>, 0, 16, 12> col192;
+const FixedCol<float, 180, 15, 32, Sched<0, 0, 0, 0>, 0, 15, 12> col180;
+const FixedCol<float, 162, 18, 16, Sched<0, 0, 0, 0>, 0, 9, 9, 2> col162;
+const FixedCol<float, 160, 16, 32, Sched<0, 0, 0, 0>, 0, 16, 10> col160;
+const FixedCol<float, 150, 15, 32, Sched<0, 0, 0, 0>, 0, 15, 10> col150;
+const FixedCol<float, 540, 45, 8, Sched<0, 0, 0, 0>, 0, 15, 6, 6> col540;
+const FixedCol<float, 125, 9, 32, Sched<0, 0, 0, 0>, 0, 25, 5> col125;
+const FixedCol<float, 729, 81, 4, Sched<0, 0, 0, 0>, 0, 9, 9, 9> col729;
+const FixedCol<float, 144, 12, 32, Sched<0, 0, 0, 0>, 0, 12, 12> col144;
+const FixedCol<float, 135, 15, 32, Sched<0, 0, 0, 0>, 0, 15, 9> col135;
+const FixedCol<float, 81, 9, 32, Sched<0, 0, 0, 0>, 0, 9, 9> col81;
 }  // namespace
 }  // namespace sfft
