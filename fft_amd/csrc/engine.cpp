@@ -415,9 +415,10 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         p2.out_elem_stride = (int64_t)n1;
         p2.pd.in_col = 1;
         p2.pd.out_col = 1;
-        // compiled column kernels (complex plans without layout hints): the
-        // pass takes the kernel's radices and tile
-        if (kind == kC2C && !(h && (h->tpw || h->fixed_variant == -2))) {
+        // compiled column kernels (plans without layout hints): the pass takes
+        // the kernel's radices and tile; it runs where the pass is a plain
+        // column pass for the direction (see run)
+        if (!(h && (h->tpw || h->fixed_variant == -2))) {
             use_fixed_columns(p1, n1, prec, kModeColRow);
             use_fixed_columns(p2, n2, prec, kModeColCol);
         }
@@ -655,9 +656,17 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         const PassPlan &pp = passes_[p];
         const size_t slots = batch * (size_t)pp.slots_per_transform;
         if (slots > 0x7fffffffull) throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
-        if (pp.fixed && pp.fixed->real == 2) {  // compiled column kernel
-            run_fixed_columns<T>(p, in, din, out, dout, batch, inverse, stream);
-            continue;
+        if (pp.fixed && pp.fixed->real == 2) {
+            // compiled column kernel where the pass is a plain column pass:
+            // complex passes, and real passes with no pre-process or rotation
+            // fused into their loads / stores for this direction
+            const PassArgs<T> m = pass_args<T>(p, inverse);
+            const bool plain = m.mode == kModeC2C ||
+                               ((m.mode == kModeR2CLoad || m.mode == kModeC2RStore) && !m.half_shift);
+            if (plain) {
+                run_fixed_columns<T>(p, in, din, out, dout, batch, inverse, stream);
+                continue;
+            }
         }
         PassArgs<T> a = pass_args<T>(p, inverse);
         a.iin.batch_stride = pp.in_from_out ? dout : din;
@@ -730,6 +739,7 @@ void Plan::run_fixed_columns(size_t p, const void *in, int64_t din, void *out, i
     if ((size_t)pp.slots_per_transform * std::min(chunk, batch) > 0x7fffffffull)
         throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
     const uint32_t flip = 0x80000000u;
+    const PassArgs<T> conj = pass_args<T>(p, inverse);  // the pass' conjugations
     // first stage with twiddles (stage 1): the pass' part of the dense table
     const int32_t tw0 = pp.pd.nstages > 1 ? pp.pd.st[1].tw_off : 0;
     for (size_t c0 = 0; c0 < batch; c0 += chunk) {
@@ -741,8 +751,8 @@ void Plan::run_fixed_columns(size_t p, const void *in, int64_t din, void *out, i
         a.out = (C2<T> *)out + (int64_t)c0 * dout;
         a.tw = (const C2<T> *)d_tw_ + tw0;
         a.total_slots = (int32_t)(cb * (size_t)pp.slots_per_transform);
-        a.flip_in = (inverse && p == 0) ? flip : 0u;
-        a.flip_out = (inverse && p + 1 == passes_.size()) ? flip : 0u;
+        a.flip_in = conj.conj_in ? flip : 0u;
+        a.flip_out = conj.conj_out ? flip : 0u;
         a.spt = make_fastdiv((uint32_t)pp.slots_per_transform);
         a.in_bs = (int32_t)ds;
         a.in_ss = (int32_t)pp.in_slot_stride;
