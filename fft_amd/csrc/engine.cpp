@@ -418,7 +418,8 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         // compiled column kernels (plans without layout hints): the pass takes
         // the kernel's radices and tile; it runs where the pass is a plain
         // column pass for the direction (see run)
-        if (!(h && (h->tpw || h->fixed_variant == -2))) {
+        // (16-bit plans asking for MFMA stages keep the runtime-planned pass)
+        if (!(h && (h->tpw || h->fixed_variant == -2 || (prec >= kF16 && h->mfma == 1)))) {
             use_fixed_columns(p1, n1, prec, kModeColRow);
             use_fixed_columns(p2, n2, prec, kModeColCol);
         }
@@ -733,7 +734,7 @@ void Plan::run_fixed_columns(size_t p, const void *in, int64_t din, void *out, i
     const PassPlan &pp = passes_[p];
     const FixedKernel &fk = *pp.fixed;
     const int mode = p == 0 ? kModeColRow : kModeColCol;
-    const size_t eb = sizeof(C2<T>);
+    const size_t eb = 2 * real_bytes(prec_);  // bytes per complex value in HBM
     const int64_t dmax = std::max(pp.in_from_out ? dout : din, dout);
     const size_t chunk = std::max<size_t>(1, ((1ull << 32) - 1) / ((size_t)dmax * eb));
     if ((size_t)pp.slots_per_transform * std::min(chunk, batch) > 0x7fffffffull)
@@ -745,10 +746,10 @@ void Plan::run_fixed_columns(size_t p, const void *in, int64_t din, void *out, i
     for (size_t c0 = 0; c0 < batch; c0 += chunk) {
         const size_t cb = std::min(chunk, batch - c0);
         FixedArgs<T> a{};
-        const C2<T> *src = pp.in_from_out ? (const C2<T> *)out : (const C2<T> *)in;
+        const char *src = pp.in_from_out ? (const char *)out : (const char *)in;
         const int64_t ds = pp.in_from_out ? dout : din;
-        a.in = src + (int64_t)c0 * ds;
-        a.out = (C2<T> *)out + (int64_t)c0 * dout;
+        a.in = (const C2<T> *)(src + (int64_t)c0 * ds * (int64_t)eb);
+        a.out = (C2<T> *)((char *)out + (int64_t)c0 * dout * (int64_t)eb);
         a.tw = (const C2<T> *)d_tw_ + tw0;
         a.total_slots = (int32_t)(cb * (size_t)pp.slots_per_transform);
         a.flip_in = conj.conj_in ? flip : 0u;

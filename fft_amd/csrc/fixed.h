@@ -67,9 +67,27 @@ struct Sched {
     static constexpr int PF_FIRST = PF_FIRST_;
 };
 
+// Element type of a compiled kernel: float / double (stored as is), or
+// Stored<StoreF16 / StoreBF16> (16-bit pairs in HBM, float arithmetic).
+template <class S>
+struct Stored {};
+template <typename E>
+struct FixedElem {
+    using T = E;
+    using S = StoreNative;
+    static constexpr int prec = sizeof(E) == 8 ? 1 : 0;
+};
+template <class S_>
+struct FixedElem<Stored<S_>> {
+    using T = float;
+    using S = S_;
+    static constexpr int prec = std::is_same<S_, StoreF16>::value ? 2 : 3;
+};
+
 template <typename T_, int N, int TPT, int TPW, int MODE, class SCH, int... Rs>
 struct Fixed {
-    using T = T_;
+    using T = typename FixedElem<T_>::T;
+    using ST = Storage<T, typename FixedElem<T_>::S>;
     static constexpr int PFS = SCH::PFS;
     static constexpr int MINW = SCH::MINW;
     static constexpr int NST = sizeof...(Rs);
@@ -81,7 +99,11 @@ struct Fixed {
     static constexpr bool COL = MODE == kModeColRow || MODE == kModeColCol;
     static constexpr bool COL_OUT = MODE == kModeColCol;
     static constexpr int STRIDE = N + (N >> kLdsPadShift) + 1;  // lds_padded(N)
-    static constexpr uint32_t EB = sizeof(C2<T>);
+    static constexpr uint32_t EB = sizeof(C2<T>);  // bytes per element in LDS
+    static constexpr uint32_t DB =                  // and in HBM
+        ST::native ? sizeof(C2<T>) : 2 * sizeof(uint16_t);
+    static __device__ __forceinline__ C2<T> ld(const char *p) { return ST::load(p, 0); }
+    static __device__ __forceinline__ void st(char *p, C2<T> v) { ST::store(p, 0, v); }
     static constexpr size_t LDS_BYTES =
         (NST > 1 || MODE != kModeC2C) ? (size_t)STRIDE * EB * TPW : 0;
 
@@ -170,28 +192,26 @@ struct Fixed {
         if constexpr (COL) {
             const uint32_t t = fdiv((uint32_t)g, a.spt);
             const uint32_t c = (uint32_t)g - t * a.spt.d;
-            const uint32_t es = (uint32_t)a.in_es * EB;
+            const uint32_t es = (uint32_t)a.in_es * DB;
             const uint32_t lane =
-                (t * (uint32_t)a.in_bs + c * (uint32_t)a.in_ss) * EB + (uint32_t)lid * es;
+                (t * (uint32_t)a.in_bs + c * (uint32_t)a.in_ss) * DB + (uint32_t)lid * es;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 if ((NB % TPT) == 0 || lid + b * TPT < NB) {
 #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        pf[b * R + r] = *reinterpret_cast<const C2<T> *>(
-                            src + (lane + (uint32_t)(b * TPT + r * NB) * es));
+                        pf[b * R + r] = ld(src + (lane + (uint32_t)(b * TPT + r * NB) * es));
                 }
             }
             return;
         }
-        const uint32_t lane = ((uint32_t)g * (uint32_t)N + (uint32_t)lid) * EB;
+        const uint32_t lane = ((uint32_t)g * (uint32_t)N + (uint32_t)lid) * DB;
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             if ((NB % TPT) == 0 || lid + b * TPT < NB) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    pf[b * R + r] = *reinterpret_cast<const C2<T> *>(
-                        src + (lane + (uint32_t)((b * TPT + r * NB) * EB)));
+                    pf[b * R + r] = ld(src + (lane + (uint32_t)((b * TPT + r * NB) * DB)));
             }
         }
     }
@@ -291,8 +311,8 @@ struct Fixed {
             char *dst = reinterpret_cast<char *>(a.out);
             const uint32_t t = fdiv((uint32_t)g, a.spt);
             const uint32_t c = (uint32_t)g - t * a.spt.d;
-            const uint32_t es = (uint32_t)a.out_es * EB;
-            const uint32_t sbase = (t * (uint32_t)a.out_bs + c * (uint32_t)a.out_ss) * EB;
+            const uint32_t es = (uint32_t)a.out_es * DB;
+            const uint32_t sbase = (t * (uint32_t)a.out_bs + c * (uint32_t)a.out_ss) * DB;
             const uint32_t mask = (1u << a.ptw_shift) - 1u;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
@@ -308,7 +328,7 @@ struct Fixed {
                             const uint32_t m = c * e;
                             y = cmul(y, cmul(a.ptw_fine[m & mask], a.ptw_coarse[m >> a.ptw_shift]));
                         }
-                        *reinterpret_cast<C2<T> *>(dst + (sbase + e * es)) = y;
+                        st(dst + (sbase + e * es), y);
                     }
                 }
             }
@@ -320,13 +340,13 @@ struct Fixed {
                 const int j = lid + b * TPT;
                 if (active && jvalid(b)) {
                     const int o = (j / NS) * NS * R + j % NS;
-                    const uint32_t lo = (sbase + (uint32_t)o) * EB;
+                    const uint32_t lo = (sbase + (uint32_t)o) * DB;
 #pragma unroll
                     for (int t = 0; t < R; ++t) {
                         C2<T> y = v[b][t];
                         flip_im(y, a.flip_out);
                         if (MODE == kModeC2R && a.half_shift) y = cmulc(y, a.rot[o + t * NS]);
-                        *reinterpret_cast<C2<T> *>(dst + (lo + (uint32_t)(t * NS * EB))) = y;
+                        st(dst + (lo + (uint32_t)(t * NS * DB)), y);
                     }
                 }
             }
@@ -437,7 +457,7 @@ struct Fixed {
             // that every XCD (blocks go round-robin over them) runs a
             // contiguous run of groups, whose shared lines meet in one L2.
             const int G = (int)gridDim.x;
-            constexpr bool NARROW = TPW * (int)EB < 128;
+            constexpr bool NARROW = TPW * (int)DB < 128;
             const int off = NARROW ? (int)xcd_tile(blockIdx.x, (unsigned)G) : (int)blockIdx.x;
             int grp = off;
             if (grp < ngroups) load0(a, pf, grp * TPW + slot0, lid0);
@@ -540,7 +560,7 @@ struct FixedC2C {
         FixedKernel k{};
         k.n = N;
         k.variant = VARIANT;
-        k.prec = sizeof(T) == 8 ? 1 : 0;
+        k.prec = FixedElem<T>::prec;
         k.real = 0;
         k.tpt = TPT;
         k.tpw = TPW;
@@ -564,7 +584,7 @@ struct FixedReal {
         FixedKernel k{};
         k.n = N;
         k.variant = VARIANT;
-        k.prec = sizeof(T) == 8 ? 1 : 0;
+        k.prec = FixedElem<T>::prec;
         k.real = 1;
         k.tpt = TPT;
         k.tpw = TPW;
@@ -592,7 +612,7 @@ struct FixedCol {
         FixedKernel k{};
         k.n = N;
         k.variant = VARIANT;
-        k.prec = sizeof(T) == 8 ? 1 : 0;
+        k.prec = FixedElem<T>::prec;
         k.real = 2;
         k.tpt = TPT;
         k.tpw = TPW;

@@ -83,6 +83,24 @@ def test_two_pass_pow2(prec, n, batch, mfma):
 
 
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
+@pytest.mark.parametrize("n,batch", [(1 << 16, 8), (1 << 17, 4), (1 << 18, 4), (1 << 20, 2)])
+@pytest.mark.parametrize("direction", [0, 1], ids=["fwd", "inv"])
+def test_compiled_columns_match_runtime_pass(prec, n, batch, direction):
+    """Two-pass 16-bit plans run compiled column kernels (column lengths
+    128-1024); they must give the bits of the runtime-planned passes (plan
+    hint variant -2), which the tests above pin to the FP64 oracle."""
+    g = torch.Generator(device="cpu").manual_seed(n + direction)
+    xs = torch.randn(batch, n, 2, generator=g).to(DT[prec]).cuda()
+    y1, y2 = torch.empty_like(xs), torch.empty_like(xs)
+    sh = torch.cuda.current_stream().cuda_stream
+    fft_amd.Plan(n, L.C2C, prec).execute_ptr(xs.data_ptr(), y1.data_ptr(), batch, direction, sh)
+    fft_amd.Plan(n, L.C2C, prec, variant=-2).execute_ptr(xs.data_ptr(), y2.data_ptr(), batch,
+                                                        direction, sh)
+    torch.cuda.synchronize()
+    assert torch.equal(y1.view(torch.int16), y2.view(torch.int16))
+
+
+@pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
 @pytest.mark.parametrize("n,batch", [(1920, 8), (3000, 8), (48, 16), (100000, 1)])
 def test_mixed_sizes(prec, n, batch):
     _, x, y = _run(n, batch, prec, 1, seed=5)
