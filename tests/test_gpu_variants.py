@@ -1,0 +1,62 @@
+"""Every compiled schedule variant of every compile-time planned length (the
+autotuner may pick any of them): forward and inverse against numpy's FFT
+(exact to rounding in double, independent of this library), and variants
+with the same radices as the default give the default's bits."""
+import numpy as np
+import pytest
+import torch
+
+import fft_amd
+from fft_amd import _lib as L
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL = {L.F32: 2e-6, L.F64: 1e-12}
+# the single-pass compiled lengths (FP32; FP64 compiles column passes only,
+# which two-pass plans exercise)
+CASES = [(1 << k, L.C2C, L.F32) for k in range(8, 15)] + [
+    (1920, L.C2C, L.F32), (3000, L.C2C, L.F32), (2048, L.R2C, L.F32)]
+
+
+def _cases():
+    return CASES
+
+
+@pytest.mark.parametrize("n,kind,prec", _cases())
+def test_every_variant(n, kind, prec):
+    nv = L.lib().sfft_plan_variants(n, kind, prec)
+    assert nv >= 1
+    batch = max(2, (1 << 16) // n)
+    rng = np.random.default_rng(n + 31 * kind + 7 * prec)
+    cdt = torch.complex64 if prec == L.F32 else torch.complex128
+    rdt = torch.float32 if prec == L.F32 else torch.float64
+    sh = torch.cuda.current_stream().cuda_stream
+    default = fft_amd.Plan(n, kind, prec, variant=0)
+    for direction in (0, 1):
+        if kind == L.C2C:
+            xa = rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))
+            x = torch.from_numpy(xa).to(cdt).cuda()
+            xd = x.to(torch.complex128).cpu().numpy()
+            want = np.fft.ifft(xd, axis=-1) * n if direction else np.fft.fft(xd, axis=-1)
+            shape, odt = (batch, n), cdt
+        elif direction == 0:
+            x = torch.from_numpy(rng.standard_normal((batch, n))).to(rdt).cuda()
+            want = po.real_fft(x.double().cpu().numpy())
+            shape, odt = (batch, n // 2), cdt
+        else:
+            xa = rng.standard_normal((batch, n // 2)) + 1j * rng.standard_normal((batch, n // 2))
+            x = torch.from_numpy(xa).to(cdt).cuda()
+            want = po.real_ifft(x.to(torch.complex128).cpu().numpy())
+            shape, odt = (batch, n), rdt
+        y0 = torch.empty(shape, dtype=odt, device="cuda")
+        default.execute_ptr(x.data_ptr(), y0.data_ptr(), batch, direction, sh)
+        for v in range(nv):
+            p = fft_amd.Plan(n, kind, prec, variant=v)
+            y = torch.full(shape, float("nan"), dtype=odt, device="cuda")
+            p.execute_ptr(x.data_ptr(), y.data_ptr(), batch, direction, sh)
+            torch.cuda.synchronize()
+            err = po.rel_l2(y.cpu().numpy(), want)
+            assert err <= TOL[prec], (n, kind, prec, v, direction, p.info(), err)
+            if p.info()["radices"] == default.info()["radices"]:
+                assert torch.equal(y, y0), (n, kind, prec, v, direction)
