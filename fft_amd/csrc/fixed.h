@@ -18,7 +18,6 @@
 namespace sfft {
 
 constexpr bool ct_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
-constexpr int ct_pad(int e) { return e + (e >> kLdsPadShift); }
 
 template <typename T>
 struct FixedArgs {
@@ -59,12 +58,15 @@ __device__ __forceinline__ void flip_im(C2<double> &x, uint32_t m) {
 //   PAIR  load twiddles two per 16-byte load (else one per load)
 //   PF_FIRST  in the prefetch stage, issue the prefetch before (1) or after
 //         (0) the next stage's twiddle loads
-template <int PFS_, int MINW_, int PAIR_ = 1, int PF_FIRST_ = 0>
+//   PADSH one LDS pad element per 2^PADSH elements (kLdsNoPad: none)
+constexpr int kLdsNoPad = 30;
+template <int PFS_, int MINW_, int PAIR_ = 1, int PF_FIRST_ = 0, int PADSH_ = kLdsPadShift>
 struct Sched {
     static constexpr int PFS = PFS_;
     static constexpr int MINW = MINW_;
     static constexpr int PAIR = PAIR_;
     static constexpr int PF_FIRST = PF_FIRST_;
+    static constexpr int PADSH = PADSH_;
 };
 
 // Element type of a compiled kernel: float / double (stored as is), or
@@ -98,7 +100,9 @@ struct Fixed {
     // coalesce across columns; the other stages keep slot-major lanes.
     static constexpr bool COL = MODE == kModeColRow || MODE == kModeColCol;
     static constexpr bool COL_OUT = MODE == kModeColCol;
-    static constexpr int STRIDE = N + (N >> kLdsPadShift) + 1;  // lds_padded(N)
+    static constexpr int PADSH = SCH::PADSH;
+    static constexpr int pad(int e) { return e + (e >> PADSH); }
+    static constexpr int STRIDE = N + (N >> PADSH) + 1;
     static constexpr uint32_t EB = sizeof(C2<T>);  // bytes per element in LDS
     static constexpr uint32_t DB =                  // and in HBM
         ST::native ? sizeof(C2<T>) : 2 * sizeof(uint16_t);
@@ -239,8 +243,8 @@ struct Fixed {
         constexpr bool FIRST = (S == 0), LAST = (S == NST - 1);
         constexpr bool SRC_PF = FIRST && MODE != kModeC2R;
         constexpr bool DST_G = LAST && MODE != kModeR2C;
-        constexpr bool SEP_R = ct_pow2(NB) || (NB % 16) == 0;
-        constexpr bool SEP_W = (NS % 16) == 0 || (ct_pow2(NS) && ct_pow2(R));
+        constexpr bool SEP_R = ct_pow2(NB) || (NB % (1 << PADSH)) == 0;
+        constexpr bool SEP_W = (NS % (1 << PADSH)) == 0 || (ct_pow2(NS) && ct_pow2(R));
         const uint32_t lsb = (uint32_t)(slot * STRIDE) * EB;
 
         auto jvalid = [&](int b) { return (NB % TPT) == 0 || lid + b * TPT < NB; };
@@ -282,16 +286,16 @@ struct Fixed {
                 const int j = lid + b * TPT;
                 if (active && jvalid(b)) {
                     if constexpr (SEP_R) {
-                        const uint32_t la = lsb + (uint32_t)ct_pad(j) * EB;
+                        const uint32_t la = lsb + (uint32_t)pad(j) * EB;
 #pragma unroll
                         for (int r = 0; r < R; ++r)
                             v[b][r] = *reinterpret_cast<const C2<T> *>(
-                                lds_c + (la + (uint32_t)(ct_pad(r * NB) * EB)));
+                                lds_c + (la + (uint32_t)(pad(r * NB) * EB)));
                     } else {
 #pragma unroll
                         for (int r = 0; r < R; ++r)
                             v[b][r] = *reinterpret_cast<const C2<T> *>(
-                                lds_c + (lsb + (uint32_t)ct_pad(j + r * NB) * EB));
+                                lds_c + (lsb + (uint32_t)pad(j + r * NB) * EB));
                     }
                 }
             }
@@ -357,16 +361,16 @@ struct Fixed {
                 if (active && jvalid(b)) {
                     const int o = (j / NS) * NS * R + j % NS;
                     if constexpr (SEP_W) {
-                        const uint32_t la = lsb + (uint32_t)ct_pad(o) * EB;
+                        const uint32_t la = lsb + (uint32_t)pad(o) * EB;
 #pragma unroll
                         for (int t = 0; t < R; ++t)
                             *reinterpret_cast<C2<T> *>(
-                                lds_c + (la + (uint32_t)(ct_pad(t * NS) * EB))) = v[b][t];
+                                lds_c + (la + (uint32_t)(pad(t * NS) * EB))) = v[b][t];
                     } else {
 #pragma unroll
                         for (int t = 0; t < R; ++t)
                             *reinterpret_cast<C2<T> *>(
-                                lds_c + (lsb + (uint32_t)ct_pad(o + t * NS) * EB)) = v[b][t];
+                                lds_c + (lsb + (uint32_t)pad(o + t * NS) * EB)) = v[b][t];
                     }
                 }
             }
@@ -409,8 +413,8 @@ struct Fixed {
                 const C2<T> ev = cmulc(erot, a.real_tw[i]);
                 const C2<T> p = cadd(odd, ev);
                 const C2<T> m = csub(odd, ev);
-                ls[ct_pad(i)] = cconj(p);
-                ls[ct_pad(ci)] = m;
+                ls[pad(i)] = cconj(p);
+                ls[pad(ci)] = m;
             }
         }
         __syncthreads();
@@ -433,7 +437,7 @@ struct Fixed {
             for (int i = (mod ? 0 : 1) + lid; i <= H / 2; i += TPT) {
                 if (real_pair_skipped(i, H, mod)) continue;
                 const int ci = mod ? (H - 1 - i) : (H - i);
-                const C2<T> zi = ls[ct_pad(i)], zc = ls[ct_pad(ci)];
+                const C2<T> zi = ls[pad(i)], zc = ls[pad(ci)];
                 const C2<T> odd = {(zi.x + zc.x) * half, (zi.y - zc.y) * half};
                 const C2<T> ev = {(zi.x - zc.x) * half, (zi.y + zc.y) * half};
                 const C2<T> e = cmul(ev, a.real_tw[i]);
