@@ -74,6 +74,30 @@ def test_strided_batches_leave_padding_untouched(n):
         assert np.all(got[:, n:] == np.complex64(9 - 9j))
 
 
+@pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
+@pytest.mark.parametrize("n", [1 << 16, 1 << 18])
+def test_strided_16bit_two_pass(prec, n):
+    """16-bit two-pass plans (compiled column passes) with batch distances
+    larger than the length: the transforms equal the contiguous ones bit for
+    bit, and the padding between them is untouched."""
+    dt = torch.float16 if prec == L.F16 else torch.bfloat16
+    batch, din, dout = 3, n + 41, n + 12
+    g = torch.Generator(device="cpu").manual_seed(n + prec)
+    x = torch.randn(batch, din, 2, generator=g).to(dt).cuda()
+    p = fft_amd.Plan(n, L.C2C, prec)
+    assert p.info()["passes"] == 2
+    for direction in (0, 1):
+        y = torch.full((batch, dout, 2), 7.0, dtype=dt, device="cuda")
+        _strided(p, x, din, y, dout, batch, direction)
+        xc = x[:, :n].contiguous()
+        yc = torch.empty_like(xc)
+        p.execute_ptr(xc.data_ptr(), yc.data_ptr(), batch, direction,
+                      torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert torch.equal(y[:, :n].view(torch.int16), yc.view(torch.int16)), direction
+        assert torch.all(y[:, n:] == 7.0)
+
+
 def test_strided_real_batches():
     n, batch = 2048, 3
     din, dout = n + 6, n // 2 + 5  # reals in, complex out
