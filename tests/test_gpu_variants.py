@@ -13,10 +13,11 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 TOL = {L.F32: 2e-6, L.F64: 1e-12}
-# the single-pass compiled lengths (FP32; FP64 compiles column passes only,
-# which two-pass plans exercise)
+# the single-pass compiled lengths (the compiled column passes are exercised
+# by two-pass plans)
 CASES = [(1 << k, L.C2C, L.F32) for k in range(8, 15)] + [
-    (1920, L.C2C, L.F32), (3000, L.C2C, L.F32), (2048, L.R2C, L.F32)]
+    (1920, L.C2C, L.F32), (3000, L.C2C, L.F32), (2048, L.R2C, L.F32)] + [
+    (1 << k, L.C2C, L.F64) for k in range(10, 14)]
 
 
 def _cases():
