@@ -55,23 +55,26 @@ def test_smallest_real_length():
     assert complex(y[0, 0]) == complex(8.0, -2.0)  # (DC, Nyquist) packed in bin 0
 
 
-@pytest.mark.parametrize("n", [4096, 1920, 777, 40000, 17])
-def test_strided_batches_leave_padding_untouched(n):
+@pytest.mark.parametrize("n,prec", [(4096, L.F32), (1920, L.F32), (777, L.F32), (40000, L.F32),
+                                    (17, L.F32), (4096, L.F64), (8192, L.F64)])
+def test_strided_batches_leave_padding_untouched(n, prec):
     """sfft_execute_strided with distances larger than the length (fixed-plan,
     runtime-planned, two-pass and chirp-z lengths)."""
     batch, din, dout = 5, n + 37, n + 11
+    cdt = np.complex64 if prec == L.F32 else np.complex128
     rng = np.random.default_rng(n)
-    xa = (rng.standard_normal((batch, din)) + 1j * rng.standard_normal((batch, din))).astype(np.complex64)
+    xa = (rng.standard_normal((batch, din)) + 1j * rng.standard_normal((batch, din))).astype(cdt)
     x = torch.from_numpy(xa).cuda()
-    y = torch.full((batch, dout), complex(9.0, -9.0), dtype=torch.complex64, device="cuda")
-    p = fft_amd.Plan(n, L.C2C, L.F32)
+    y = torch.full((batch, dout), complex(9.0, -9.0), dtype=x.dtype, device="cuda")
+    p = fft_amd.Plan(n, L.C2C, prec)
+    tol = 1e-6 if prec == L.F32 else 1e-13
     for direction in (0, 1):
         _strided(p, x, din, y, dout, batch, direction)
         torch.cuda.synchronize()
         got = y.cpu().numpy()
         want = po.fft(xa[:, :n].astype(np.complex128), inverse=bool(direction))
-        assert po.rel_l2(got[:, :n], want) <= 1e-6, (n, direction)
-        assert np.all(got[:, n:] == np.complex64(9 - 9j))
+        assert po.rel_l2(got[:, :n], want) <= tol, (n, direction)
+        assert np.all(got[:, n:] == cdt(9 - 9j))
 
 
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
@@ -180,20 +183,23 @@ def test_largest_lengths_single_bin(n, prec):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize("n,kind", [(4096, L.C2C), (777, L.C2C), (17, L.C2C), (2048, L.R2C),
-                                    (6000, L.R2C_HALFSHIFT), (34, L.R2C)])
-def test_in_place_one_pass_and_chirp_z(n, kind):
+@pytest.mark.parametrize("n,kind,prec", [(4096, L.C2C, L.F32), (777, L.C2C, L.F32),
+                                         (17, L.C2C, L.F32), (2048, L.R2C, L.F32),
+                                         (6000, L.R2C_HALFSHIFT, L.F32), (34, L.R2C, L.F32),
+                                         (4096, L.C2C, L.F64), (1024, L.C2C, L.F64)])
+def test_in_place_one_pass_and_chirp_z(n, kind, prec):
     """in == out with the same layout: results equal the out-of-place ones."""
-    p = fft_amd.Plan(n, kind, L.F32)
+    p = fft_amd.Plan(n, kind, prec)
     assert p.info()["in_place"] == 1
     batch = 6
+    cdt = torch.complex64 if prec == L.F32 else torch.complex128
+    rdt = torch.float32 if prec == L.F32 else torch.float64
     sh = torch.cuda.current_stream().cuda_stream
     for d in (0, 1):
         if kind == L.C2C or d == 1:
-            x = torch.randn(batch, n if kind == L.C2C else n // 2, dtype=torch.complex64,
-                            device="cuda")
+            x = torch.randn(batch, n if kind == L.C2C else n // 2, dtype=cdt, device="cuda")
         else:
-            x = torch.randn(batch, n, dtype=torch.float32, device="cuda")
+            x = torch.randn(batch, n, dtype=rdt, device="cuda")
         nbytes = x.numel() * x.element_size()
         want = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
         p.execute_ptr(x.data_ptr(), want.data_ptr(), batch, d, sh)
