@@ -338,7 +338,9 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     const bool force_one = h && h->split == 1;
     bool ok = false;
     // A compile-time planned kernel fixes the radices of this length.
-    const FixedKernel *fk = (h && (!h->radices.empty() || h->fixed_variant == -2))
+    // (16-bit plans asking for MFMA stages keep the runtime-planned pass)
+    const FixedKernel *fk = (h && (!h->radices.empty() || h->fixed_variant == -2 ||
+                                   (prec >= kF16 && h->mfma == 1)))
                                 ? nullptr
                                 : find_fixed((int)cn_, prec, kind != kC2C ? 1 : 0,
                                              h ? h->fixed_variant : -1);
@@ -702,15 +704,15 @@ void Plan::run_fixed(const void *in, void *out, size_t batch, int inverse,
     const PassPlan &pp = passes_[0];
     const FixedKernel &fk = *pp.fixed;
     const int mode = kind_ == kC2C ? kModeC2C : (inverse ? kModeC2R : kModeR2C);
-    const size_t eb = sizeof(C2<T>);
+    const size_t eb = 2 * real_bytes(prec_);  // bytes per complex value in HBM
     const size_t per = cn_ * eb;  // bytes per transform (input and output)
     const size_t chunk = std::max<size_t>(1, ((1ull << 32) - 1) / per);
     const uint32_t flip = inverse ? 0x80000000u : 0u;
     for (size_t c0 = 0; c0 < batch; c0 += chunk) {
         const size_t cb = std::min(chunk, batch - c0);
         FixedArgs<T> a{};
-        a.in = (const C2<T> *)in + c0 * cn_;
-        a.out = (C2<T> *)out + c0 * cn_;
+        a.in = (const C2<T> *)((const char *)in + c0 * per);
+        a.out = (C2<T> *)((char *)out + c0 * per);
         a.tw = (const C2<T> *)d_tw_;
         a.real_tw = (const C2<T> *)d_real_tw_;
         a.rot = (const C2<T> *)d_rot_;

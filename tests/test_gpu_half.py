@@ -83,12 +83,14 @@ def test_two_pass_pow2(prec, n, batch, mfma):
 
 
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
-@pytest.mark.parametrize("n,batch", [(1 << 16, 8), (1 << 17, 4), (1 << 18, 4), (1 << 20, 2)])
+@pytest.mark.parametrize("n,batch", [(1024, 64), (4096, 16), (16384, 4), (1 << 16, 8),
+                                     (1 << 17, 4), (1 << 18, 4), (1 << 20, 2)])
 @pytest.mark.parametrize("direction", [0, 1], ids=["fwd", "inv"])
-def test_compiled_columns_match_runtime_pass(prec, n, batch, direction):
-    """Two-pass 16-bit plans run compiled column kernels (column lengths
-    128-1024); they must give the bits of the runtime-planned passes (plan
-    hint variant -2), which the tests above pin to the FP64 oracle."""
+def test_compiled_kernels_match_runtime_pass(prec, n, batch, direction):
+    """16-bit plans run compiled kernels (single pass: 1024, 4096, 16384; two
+    passes: column lengths 128-1024); they must give the bits of the
+    runtime-planned passes (plan hint variant -2), which the tests above pin
+    to the FP64 oracle."""
     g = torch.Generator(device="cpu").manual_seed(n + direction)
     xs = torch.randn(batch, n, 2, generator=g).to(DT[prec]).cuda()
     y1, y2 = torch.empty_like(xs), torch.empty_like(xs)
