@@ -83,12 +83,13 @@ def test_two_pass_pow2(prec, n, batch, mfma):
 
 
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
-@pytest.mark.parametrize("n,batch", [(1024, 64), (4096, 16), (16384, 4), (1 << 16, 8),
-                                     (1 << 17, 4), (1 << 18, 4), (1 << 20, 2)])
+@pytest.mark.parametrize("n,batch", [(256, 64), (1024, 64), (2048, 32), (4096, 16), (16384, 4),
+                                     (1 << 16, 8), (1 << 17, 4), (1 << 18, 4), (1 << 20, 2)])
 @pytest.mark.parametrize("direction", [0, 1], ids=["fwd", "inv"])
 def test_compiled_kernels_match_runtime_pass(prec, n, batch, direction):
-    """16-bit plans run compiled kernels (single pass: 1024, 4096, 16384; two
-    passes: column lengths 128-1024); they must give the bits of the
+    """16-bit plans run compiled kernels (single pass: powers of two 256-16384;
+    two passes: column lengths 128-1024); where the radices match the
+    runtime planner's (all these lengths) they must give the bits of the
     runtime-planned passes (plan hint variant -2), which the tests above pin
     to the FP64 oracle."""
     g = torch.Generator(device="cpu").manual_seed(n + direction)
@@ -100,6 +101,22 @@ def test_compiled_kernels_match_runtime_pass(prec, n, batch, direction):
                                                         direction, sh)
     torch.cuda.synchronize()
     assert torch.equal(y1.view(torch.int16), y2.view(torch.int16))
+
+
+@pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])
+@pytest.mark.parametrize("n,batch", [(512, 64), (8192, 8)])
+@pytest.mark.parametrize("direction", [0, 1], ids=["fwd", "inv"])
+def test_compiled_own_radices(prec, n, batch, direction):
+    """Compiled 16-bit kernels whose radices differ from the runtime
+    planner's (512: 16 8 4, 8192: 16 16 8 4): against the FP64 oracle, within
+    the storage rounding bound of the single-pass tests."""
+    p, x, y = _run(n, batch, prec, direction, seed=11)
+    assert p.info()["passes"] == 1
+    want = po.fft(x, inverse=bool(direction))
+    err = po.rel_l2(y, want)
+    ideal = po.rel_l2(_round_to(want, prec), want)
+    assert err <= TOL[prec]
+    assert err <= 1.3 * ideal + 1e-6, (err, ideal)
 
 
 @pytest.mark.parametrize("prec", [L.F16, L.BF16], ids=["f16", "bf16"])

@@ -18,7 +18,7 @@ TOL = {L.F32: 2e-6, L.F64: 1e-12}
 CASES = [(1 << k, L.C2C, L.F32) for k in range(8, 15)] + [
     (1920, L.C2C, L.F32), (3000, L.C2C, L.F32)] + [
     (n, L.R2C, L.F32) for n in (1024, 2048, 4096, 8192, 16384)] + [
-    (1 << k, L.C2C, L.F64) for k in range(10, 14)] + [
+    (1 << k, L.C2C, L.F64) for k in range(8, 14)] + [
     (n, L.R2C, L.F64) for n in (2048, 4096, 8192)]
 
 
