@@ -20,13 +20,25 @@ UnevenBatch::UnevenBatch(const sfft_batch_item *items, size_t count, int prec)
                             ": lengths with prime factors > 13 run through sfft_execute only");
     }
     const size_t lds_cap = device_info().lds_bytes - kLdsReserve;
-    if (prec == kF64)
-        build<double>(lds_cap);
-    else
-        build<float>(lds_cap);
+    check_hip(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "batch event");
+    try {
+        if (prec == kF64)
+            build<double>(lds_cap);
+        else
+            build<float>(lds_cap);
+    } catch (...) {
+        release();
+        throw;
+    }
 }
 
-UnevenBatch::~UnevenBatch() {
+UnevenBatch::~UnevenBatch() { release(); }
+
+void UnevenBatch::release() {
+    // a launch still in flight reads the unit list and the counters
+    if (done_ && launched_) (void)hipEventSynchronize(done_);
+    if (done_) (void)hipEventDestroy(done_);
+    done_ = nullptr;
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (d_units_) (void)hipFree(d_units_);
@@ -34,6 +46,12 @@ UnevenBatch::~UnevenBatch() {
     if (d_counters_) (void)hipFree(d_counters_);
     if (d_gather_in_) (void)hipFree(d_gather_in_);
     if (d_gather_out_) (void)hipFree(d_gather_out_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    d_units_ = nullptr;
+    d_table_ = nullptr;
+    d_counters_ = nullptr;
+    d_gather_in_ = d_gather_out_ = nullptr;
 }
 
 template <typename T>
@@ -166,7 +184,10 @@ void UnevenBatch::launch(const void *in_arena, void *out_arena, hipStream_t stre
 void UnevenBatch::execute_persistent(const void *in_arena, void *out_arena, hipStream_t stream) {
     if (!in_arena || !out_arena) throw Error(SFFT_ERR_INVALID_ARG, "NULL arena");
     if (in_arena == out_arena) throw Error(SFFT_ERR_ALIASING, "arenas must differ");
+    std::lock_guard<std::mutex> lock(mu_);
     if (!graph_exec_ || graph_in_ != in_arena || graph_out_ != out_arena) {
+        // the previous launch may still run the graph about to be replaced
+        if (launched_) check_hip(hipEventSynchronize(done_), "previous batch launch");
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
         graph_exec_ = nullptr;
@@ -183,7 +204,12 @@ void UnevenBatch::execute_persistent(const void *in_arena, void *out_arena, hipS
         graph_in_ = in_arena;
         graph_out_ = out_arena;
     }
+    // the work queue is shared by every launch of this batch: start after the
+    // previous launch has drained it (and reset its counters)
+    if (launched_) check_hip(hipStreamWaitEvent(stream, done_, 0), "order batch launches");
     check_hip(hipGraphLaunch(graph_exec_, stream), "graph launch");
+    check_hip(hipEventRecord(done_, stream), "batch launch event");
+    launched_ = true;
 }
 
 void UnevenBatch::execute_per_size(const void *in_arena, void *out_arena, hipStream_t stream) {

@@ -5,6 +5,7 @@
 #include <sfft.h>
 
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "engine.h"
@@ -20,7 +21,10 @@ class UnevenBatch {
     UnevenBatch &operator=(const UnevenBatch &) = delete;
 
     // One persistent launch over every transform of every item, replayed
-    // from a captured hipGraph (re-captured when the arenas change).
+    // from a captured hipGraph (re-captured when the arenas change).  The
+    // launch's work queue lives in the batch, so launches of one batch are
+    // serialised on the device: each waits for the previous one (on any
+    // stream, from any thread) before it starts.
     void execute_persistent(const void *in_arena, void *out_arena, hipStream_t stream);
     // The plain per-size loop: one batched launch sequence per item.
     void execute_per_size(const void *in_arena, void *out_arena, hipStream_t stream);
@@ -32,6 +36,7 @@ class UnevenBatch {
     template <typename T>
     void build(size_t lds_cap);
     void launch(const void *in_arena, void *out_arena, hipStream_t stream);
+    void release();
 
     int prec_;
     std::vector<sfft_batch_item> items_;
@@ -44,7 +49,11 @@ class UnevenBatch {
     size_t nunits_ = 0;
     unsigned grid_ = 0;
     size_t lds_bytes_ = 0;
-    // captured launch
+    // captured launch; mu_ guards the graph and the ordering event, which
+    // marks the end of the last launch (launched_: recorded at least once)
+    std::mutex mu_;
+    hipEvent_t done_ = nullptr;
+    bool launched_ = false;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     const void *graph_in_ = nullptr;

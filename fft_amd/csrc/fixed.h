@@ -521,17 +521,9 @@ int fixed_variant_count(int n, int prec, int real);
 
 template <class F>
 hipError_t fixed_launch_one(const void *args, unsigned grid, hipStream_t stream) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_fixed<F>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024 - kLdsReserve);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return e;
-        }
-        attr = true;
-    }
+    static std::once_flag lds_once;
+    static hipError_t lds_err = hipSuccess;
+    if (allow_max_lds(lds_once, lds_err, (const void *)k_fixed<F>) != hipSuccess) return lds_err;
     hipLaunchKernelGGL((k_fixed<F>), dim3(grid), dim3(F::THREADS), F::LDS_BYTES, stream,
                        *reinterpret_cast<const FixedArgs<typename F::T> *>(args));
     return hipGetLastError();

@@ -18,17 +18,9 @@ __global__ void __launch_bounds__(kMaxBlock<T>()) k_pass(PassArgs<T> a) {
 template <typename T, int EPT>
 static hipError_t launch_one(const PassArgs<T> &a, unsigned grid, unsigned block, size_t lds_bytes,
                              hipStream_t stream) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pass<T, EPT>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024 - kLdsReserve);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return e;
-        }
-        attr = true;
-    }
+    static std::once_flag lds_once;
+    static hipError_t lds_err = hipSuccess;
+    if (allow_max_lds(lds_once, lds_err, (const void *)k_pass<T, EPT>) != hipSuccess) return lds_err;
     hipLaunchKernelGGL((k_pass<T, EPT>), dim3(grid), dim3(block), lds_bytes, stream, a);
     return hipGetLastError();
 }

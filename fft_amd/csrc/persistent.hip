@@ -76,17 +76,9 @@ hipError_t launch_persistent(const WorkUnit *units, int nunits, const PassArgs<T
                              unsigned *counters, const void *in_arena, void *out_arena,
                              unsigned grid, size_t lds_bytes, hipStream_t stream) {
     constexpr int E = persistent_ept<T>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_persistent<T, E>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024 - kLdsReserve);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return e;
-        }
-        attr = true;
-    }
+    static std::once_flag lds_once;
+    static hipError_t lds_err = hipSuccess;
+    if (allow_max_lds(lds_once, lds_err, (const void *)k_persistent<T, E>) != hipSuccess) return lds_err;
     hipLaunchKernelGGL((k_persistent<T, E>), dim3(grid), dim3(persistent_block<T>()), lds_bytes,
                        stream, units, nunits, table, gather_in, gather_out, counters,
                        (const char *)in_arena,

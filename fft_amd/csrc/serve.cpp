@@ -234,11 +234,21 @@ class Server {
         char *in = pool_->base() + r->in_off;
         char *out = pool_->base() + r->out_off;
         const size_t reals = r->in_bytes / (cfg_.precision == kF64 ? 8 : 4);
-        check_hip(fill_synthetic(in, reals, cfg_.precision == kF64, 0, r->r.seed, s), "fill");
-        r->plan->execute(in, r->plan->default_in_dist(r->r.direction), out,
-                         r->plan->default_out_dist(r->r.direction), r->r.batch, r->r.direction, s);
-        r->ev = get_event();
-        check_hip(hipEventRecord(r->ev, s), "event record");
+        try {
+            check_hip(fill_synthetic(in, reals, cfg_.precision == kF64, 0, r->r.seed, s),
+                      "fill");
+            r->plan->execute(in, r->plan->default_in_dist(r->r.direction), out,
+                             r->plan->default_out_dist(r->r.direction), r->r.batch,
+                             r->r.direction, s);
+            r->ev = get_event();
+            check_hip(hipEventRecord(r->ev, s), "event record");
+        } catch (...) {
+            // part of the request may already be queued on s: its pool ranges
+            // are released by the caller only once that work has finished
+            (void)hipStreamSynchronize(s);
+            (void)hipGetLastError();
+            throw;
+        }
         if (cfg_.plain) check_hip(hipStreamSynchronize(s), "plain sync");
     }
 

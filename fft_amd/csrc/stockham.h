@@ -7,11 +7,25 @@
 // The inverse transform is conj(FFT(conj(x))): conj on load and on store.
 #pragma once
 
+#include <mutex>
+
 #include "dft.h"
 #include "plan_desc.h"
 #include "storage.h"
 
 namespace sfft {
+
+// Raises a kernel's dynamic LDS limit to what one CU holds, once per kernel
+// and process: the first caller sets the attribute, every caller (on any
+// thread) gets that call's result.
+inline hipError_t allow_max_lds(std::once_flag &once, hipError_t &err, const void *kernel) {
+    std::call_once(once, [&] {
+        err = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024 - kLdsReserve);
+        if (err != hipSuccess) (void)hipGetLastError();
+    });
+    return err;
+}
 
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f) {
     if (f.d <= 1) return n;

@@ -200,7 +200,9 @@ int sfft_batch_create(sfft_batch *batch, const sfft_batch_item *items, size_t co
                       int precision);
 int sfft_batch_destroy(sfft_batch batch);
 /* One persistent launch over every item: in_arena/out_arena are device
- * base pointers for the item offsets. */
+ * base pointers for the item offsets.  Launches of one batch share its work
+ * queue, so they are serialised on the device: a launch waits for the
+ * previous launch of the same batch (on any stream, from any thread). */
 int sfft_batch_execute(sfft_batch batch, const void *in_arena, void *out_arena, void *stream);
 /* Same work as one sfft_execute per item group (the plain per-size loop). */
 int sfft_batch_execute_per_size(sfft_batch batch, const void *in_arena, void *out_arena,

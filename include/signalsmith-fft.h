@@ -81,16 +81,20 @@ class Engine {
     Engine &operator=(const Engine &) = delete;
     ~Engine() { release(); }
 
+    // Builds the new plan and buffers aside and takes them only once all
+    // succeeded: on an error (FFTError) the engine keeps its previous state.
     void plan(size_t n, int kind, int prec, size_t in_bytes, size_t out_bytes) {
-        release();
-        if (n == 0) return;
-        check(sfft_plan_create(&plan_, n, kind, prec));
-        check(sfft_malloc_device(&din_, in_bytes));
-        check(sfft_malloc_device(&dout_, out_bytes));
-        check(sfft_malloc_host(&hin_, in_bytes));
-        check(sfft_malloc_host(&hout_, out_bytes));
-        in_bytes_ = in_bytes;
-        out_bytes_ = out_bytes;
+        Engine next;
+        if (n) {
+            check(sfft_plan_create(&next.plan_, n, kind, prec));
+            check(sfft_malloc_device(&next.din_, in_bytes));
+            check(sfft_malloc_device(&next.dout_, out_bytes));
+            check(sfft_malloc_host(&next.hin_, in_bytes));
+            check(sfft_malloc_host(&next.hout_, out_bytes));
+            next.in_bytes_ = in_bytes;
+            next.out_bytes_ = out_bytes;
+        }
+        swap(next);
     }
     bool ready() const { return plan_ != nullptr; }
     void *host_in() const { return hin_; }
@@ -108,6 +112,15 @@ class Engine {
     }
 
   private:
+    void swap(Engine &o) {
+        std::swap(plan_, o.plan_);
+        std::swap(din_, o.din_);
+        std::swap(dout_, o.dout_);
+        std::swap(hin_, o.hin_);
+        std::swap(hout_, o.hout_);
+        std::swap(in_bytes_, o.in_bytes_);
+        std::swap(out_bytes_, o.out_bytes_);
+    }
     void release() {
         if (plan_) sfft_plan_destroy(plan_);
         if (din_) sfft_free_device(din_);

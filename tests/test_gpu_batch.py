@@ -47,6 +47,32 @@ def test_uneven_batch_matches_per_size_and_oracle():
         assert err <= TOL32, (n, d, err)
 
 
+def test_full_uneven_workload():
+    """The whole 20000-transform uneven workload the bench times (seed 3, the
+    same input fill): bitwise equal to the per-size loop, and the oracle on a
+    sample holding 24 two-pass transforms and 16 one-pass ones."""
+    sizes, dirs = workloads.uneven_batch(seed=3, count=20000)
+    b, xin = _make(sizes, dirs, seed=3)
+    out_p = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    out_s = torch.zeros_like(out_p)
+    b.execute(xin, out_p)
+    b.execute_per_size(xin, out_s)
+    torch.cuda.synchronize()
+    assert torch.equal(out_p, out_s), "persistent mix differs from the per-size loop"
+    passes = {int(n): fft_amd.Plan(int(n)).info()["passes"] for n in set(sizes.tolist())}
+    two = [i for i in range(len(sizes)) if passes[int(sizes[i])] == 2]
+    one = [i for i in range(len(sizes)) if passes[int(sizes[i])] == 1]
+    assert len(two) >= 24, len(two)
+    rng = np.random.default_rng(1)
+    pick = list(rng.choice(two, 24, replace=False)) + list(rng.choice(one, 16, replace=False))
+    for i in pick:
+        n, d = int(sizes[i]), int(dirs[i])
+        x = _view_c64(xin, b.in_offsets[i], n).cpu().numpy()
+        y = _view_c64(out_p, b.out_offsets[i], n).cpu().numpy()
+        err = po.rel_l2(y, po.fft(x.astype(np.complex128), inverse=bool(d)))
+        assert err <= TOL32, (i, n, d, passes[n], err)
+
+
 def test_same_result_alone_and_in_the_mix():
     sizes, dirs = workloads.uneven_batch(seed=5, count=600)
     b, xin = _make(sizes, dirs, seed=5)
@@ -76,6 +102,29 @@ def test_graph_replays_are_stable():
         torch.cuda.synchronize()
         assert torch.equal(out, ref), f"replay {r} differs"
         assert torch.isfinite(junk)
+
+
+def test_back_to_back_on_two_streams():
+    """Launches of one batch on two streams share its work queue: the library
+    orders them, so every launch completes the whole batch."""
+    sizes, dirs = workloads.uneven_batch(seed=17, count=500)
+    b, xin = _make(sizes, dirs, seed=17)
+    ref = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    b.execute_per_size(xin, ref)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    o1, o2 = torch.empty_like(ref), torch.empty_like(ref)
+    for r in range(3):
+        o1.zero_()
+        o2.zero_()
+        torch.cuda.synchronize()
+        b.execute(xin, o1, stream=s1)
+        b.execute(xin, o1, stream=s2)
+        b.execute(xin, o2, stream=s1)  # other arena: re-captured graph
+        b.execute(xin, o2, stream=s2)
+        torch.cuda.synchronize()
+        assert torch.equal(o1, ref), f"round {r}: first arena differs"
+        assert torch.equal(o2, ref), f"round {r}: second arena differs"
 
 
 def test_mixed_kinds_in_one_launch():

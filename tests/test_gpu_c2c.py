@@ -233,6 +233,24 @@ def test_two_pass_double():
     assert po.rel_l2(_run(n, x, dtype=torch.complex128), po.fft(x)) <= 1e-12
 
 
+@pytest.mark.parametrize("n,batch", [(2 ** 20, 2), (2 ** 24, 1)])
+def test_two_pass_double_long(n, batch):
+    """FP64 at the long-transform lengths (2^20 x 2, 2^24 x 1), forward and
+    inverse, against the FP64 oracle (the unnormalised pair of
+    signalsmith-fft.h:295-315, :374-386)."""
+    x = synth.complex_normal(n + 20, (batch, n), np.complex128)
+    f = fft_amd.FFT(n, dtype=torch.complex128)
+    assert f.plan.info()["passes"] == 2
+    y = _run(n, x, dtype=torch.complex128)
+    err = po.rel_l2(y, po.fft(x))
+    print("F64", n, "forward rel L2", err)
+    assert err <= 1e-12, err
+    z = _run(n, y, inverse=True, dtype=torch.complex128)
+    err = po.rel_l2(z, po.ifft(y))
+    print("F64", n, "inverse rel L2", err)
+    assert err <= 1e-12, err
+
+
 def test_aliasing_rejected_in_place_allowed():
     # partial overlap: rejected; exactly in place on a one-pass length: the
     # same values as out of place (the reference corrupts in place)

@@ -138,14 +138,9 @@ def test_real_setters_mirror_reference():
     rf = fft_amd.RealFFT(2)
     for n in (4, 100, 1000, 2048, 3000):
         assert rf.setSize(n) == t["real_set"][n - 1]  # returns the half size
-    from fft_amd import _lib as L
     for n in (5, 7, 100, 1000, 2047):
-        want = int(t["real_setmin"][n - 1])
-        if L.lib().sfft_size_supported(2 * want, L.R2C):
-            assert rf.setSizeMinimum(n) == want
-        else:  # e.g. 2047 -> 2050 (half 1025 = 5^2 * 41): not planned yet
-            with pytest.raises(L.SfftError):
-                rf.setSizeMinimum(n)
+        # every length is planned (2047 -> 2050, half 1025 = 5^2 * 41, runs chirp-z)
+        assert rf.setSizeMinimum(n) == t["real_setmin"][n - 1]
         assert rf.setSizeMaximum(n) == t["real_setmax"][n - 1]
 
 

@@ -60,6 +60,45 @@ def test_same_result_alone_and_in_the_mix():
             assert po.rel_l2(got[0], want) <= 1e-6, (n, k, d)
 
 
+def test_same_result_at_the_largest_requests():
+    """Requests at the top of the serving range (N up to 2^18, batches up to
+    512, all three kinds) served together equal the plain plan bitwise; the
+    first transform of each is checked against the oracle."""
+    reqs = [(1 << 18, L.C2C, 0, 512), (1 << 18, L.R2C, 0, 300), (3 << 16, L.C2C, 1, 200),
+            (1 << 18, L.R2C, 1, 64), (256, L.C2C, 1, 512), (3 << 12, L.C2C, 0, 512),
+            (5 ** 7, L.C2C, 1, 100), (2 ** 17, L.R2C, 0, 512)]
+    with Server(budget_bytes=16 << 30, max_inflight=8, streams=4) as srv:
+        for i, (n, k, d, b) in enumerate(reqs):
+            srv.submit(i, n, k, d, b, seed=2000 + i, retain=True)
+        srv.drain()
+        rec = srv.poll()
+        assert len(rec) == len(reqs) and (rec["status"] == 0).all()
+        for i, (n, k, d, b) in enumerate(reqs):
+            x, y = _alone(n, k, d, b, 2000 + i)
+            got = srv.fetch(i, y.nbytes).view(y.dtype).reshape(y.shape)
+            srv.release(i)
+            assert np.array_equal(got, y), (n, k, d, b)
+            if k == L.C2C:
+                want = po.fft(x[0].astype(np.complex128), inverse=bool(d))
+            elif d == 0:
+                want = po.real_fft(x[0].astype(np.float64))
+            else:
+                want = po.real_ifft(x[0].astype(np.complex128))
+            assert po.rel_l2(got[0], want) <= 1e-6, (n, k, d)
+            del x, y, got
+
+
+def test_poisson_run_full_range():
+    """Poisson arrivals over the whole serving range: sizes log-uniform in
+    [256, 2^18], batches 1..512, a 64 GB budget."""
+    with Server(budget_bytes=64 << 30, max_inflight=64, streams=4) as srv:
+        rec = srv.run_poisson(rate=2000, duration=0.25, n_min=256, n_max=1 << 18,
+                              batch_max=512, seed=5)
+    s = latency_summary(rec)
+    assert s["requests"] > 300 and s["failed"] == 0, s
+    assert s["p50_ms"] > 0 and s["p99_ms"] >= s["p50_ms"]
+
+
 def test_backpressure_under_small_budget():
     # 32 requests of 32 MiB each through a 96 MiB pool: at most a few fit at once
     with Server(budget_bytes=96 << 20, max_inflight=16) as srv:
