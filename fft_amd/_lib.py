@@ -22,7 +22,7 @@ SFFT_ERR_NOT_READY = -6
 SFFT_ERR_BUDGET = -7
 
 C2C, R2C, R2C_HALFSHIFT = 0, 1, 2
-F32, F64, F16, BF16 = 0, 1, 2, 3
+F32, F64, F16, BF16, F32X = 0, 1, 2, 3, 4
 FORWARD, INVERSE = 0, 1
 
 # Every symbol include/sfft.h declares (checked by tests/test_capi_exports.py).

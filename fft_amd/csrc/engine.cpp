@@ -153,9 +153,9 @@ static size_t bluestein_length(size_t n) {
     return best;
 }
 
-static size_t elem_bytes(int prec) { return prec == kF64 ? 16 : 8; }
+static size_t elem_bytes(int prec) { return arith_f64(prec) ? 16 : 8; }
 static int max_block(int prec) {
-    return prec >= kF16 ? kMaxBlock16 : (prec == kF64 ? kMaxBlock<double>() : kMaxBlock<float>());
+    return stored_16(prec) ? kMaxBlock16 : (arith_f64(prec) ? kMaxBlock<double>() : kMaxBlock<float>());
 }
 
 // Dynamic LDS a layout may use: the device limit less room for the kernels'
@@ -190,7 +190,7 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
     int ept = 0;
     const int epts[2] = {kEptSmall, kEptLarge};
     for (int e : epts) {
-        if (prec == kF64 && e > kEptSmall) continue;
+        if (arith_f64(prec) && e > kEptSmall) continue;
         if (h && h->ept && h->ept != e) continue;
         if (h && !h->radices.empty()) {
             int prod = 1;
@@ -198,7 +198,7 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
             if (prod != n) return false;
             radices = h->radices;
             if (threads_for(n, radices, e) > max_block(prec)) continue;
-        } else if (!choose_radices(n, e, radices, prec == kF64 ? 16 : 25, max_block(prec))) {
+        } else if (!choose_radices(n, e, radices, arith_f64(prec) ? 16 : 25, max_block(prec))) {
             continue;
         }
         ept = e;
@@ -234,9 +234,9 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
     std::vector<int> radices;
     int ept = 0;
     for (int e : {kEptSmall, kEptLarge}) {
-        if (prec == kF64 && e > kEptSmall) continue;
+        if (arith_f64(prec) && e > kEptSmall) continue;
         if (h && h->ept && h->ept != e) continue;
-        if (choose_radices(n, e, radices, prec == kF64 ? 16 : 25, max_block(prec))) {
+        if (choose_radices(n, e, radices, arith_f64(prec) ? 16 : 25, max_block(prec))) {
             ept = e;
             break;
         }
@@ -301,19 +301,20 @@ static void enable_mfma(PassPlan &pp) {
 Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind), prec_(prec) {
     if (kind != kC2C && kind != kR2C && kind != kR2CHalf)
         throw Error(SFFT_ERR_INVALID_ARG, "unknown transform kind");
-    if (prec != kF32 && prec != kF64 && prec != kF16 && prec != kBF16)
+    if (prec != kF32 && prec != kF64 && prec != kF16 && prec != kBF16 && prec != kF32X)
         throw Error(SFFT_ERR_INVALID_ARG, "unknown precision");
-    if (prec >= kF16 && kind != kC2C)
+    if (mixed_storage(prec) && kind != kC2C)
         throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
-                    "16-bit storage runs complex transforms only (real kinds: F32 or F64)");
+                    "16-bit storage and F32X run complex transforms only (real kinds: F32 or "
+                    "F64)");
     if (n == 0) throw Error(SFFT_ERR_INVALID_ARG, "size 0");
     if (kind != kC2C && (n % 2))
         throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "real transforms need an even size");
     cn_ = kind == kC2C ? n : n / 2;
     if (!supported_length(cn_)) {
-        if (prec >= kF16)
+        if (mixed_storage(prec))
             throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
-                        "16-bit storage: only lengths with factors 2, 3, 5, 7, 11, 13");
+                        "16-bit storage and F32X: only lengths with factors 2, 3, 5, 7, 11, 13");
         if (cn_ > kMaxBluestein)
             throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "size " + std::to_string(n) + " too large");
         bs_m_ = bluestein_length(cn_);
@@ -340,7 +341,7 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     // A compile-time planned kernel fixes the radices of this length.
     // (16-bit plans asking for MFMA stages keep the runtime-planned pass)
     const FixedKernel *fk = (h && (!h->radices.empty() || h->fixed_variant == -2 ||
-                                   (prec >= kF16 && h->mfma == 1)))
+                                   (stored_16(prec) && h->mfma == 1)))
                                 ? nullptr
                                 : find_fixed((int)cn_, prec, kind != kC2C ? 1 : 0,
                                              h ? h->fixed_variant : -1);
@@ -421,7 +422,7 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
         // the kernel's radices and tile; it runs where the pass is a plain
         // column pass for the direction (see run)
         // (16-bit plans asking for MFMA stages keep the runtime-planned pass)
-        if (!(h && (h->tpw || h->fixed_variant == -2 || (prec >= kF16 && h->mfma == 1)))) {
+        if (!(h && (h->tpw || h->fixed_variant == -2 || (stored_16(prec) && h->mfma == 1)))) {
             use_fixed_columns(p1, n1, prec, kModeColRow);
             use_fixed_columns(p2, n2, prec, kModeColCol);
         }
@@ -431,10 +432,10 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     // 16-bit storage: radix-16 stages on MFMA when asked for (measured slower
     // than float butterflies here: the operand splits and the larger register
     // footprint outweigh the matrix-core arithmetic in an HBM-bound pass)
-    if (prec >= kF16 && h && h->mfma == 1) {
+    if (stored_16(prec) && h && h->mfma == 1) {
         for (auto &pp : passes_) enable_mfma(pp);
     }
-    if (prec == kF64)
+    if (arith_f64(prec))
         build_tables<double>();
     else
         build_tables<float>();
@@ -679,10 +680,16 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         a.total_slots = (int32_t)slots;
         const unsigned grid = (unsigned)((slots + pp.pd.tpw - 1) / pp.pd.tpw);
         if constexpr (std::is_same<T, float>::value) {
-            if (prec_ >= kF16) {
+            if (stored_16(prec_)) {
                 check_hip(launch_pass_16(a, prec_ == kF16 ? kStorageF16 : kStorageBF16, pp.ept,
                                          grid, pp.block, pp.lds_bytes, stream),
                           "pass launch");
+                continue;
+            }
+        }
+        if constexpr (std::is_same<T, double>::value) {
+            if (prec_ == kF32X) {
+                check_hip(launch_pass_f32x(a, grid, pp.block, pp.lds_bytes, stream), "pass launch");
                 continue;
             }
         }
@@ -801,7 +808,7 @@ void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, s
     }
     if (in_dist < default_in_dist(inverse) || out_dist < default_out_dist(inverse))
         throw Error(SFFT_ERR_INVALID_ARG, "batch distance shorter than one transform");
-    if (prec_ == kF64)
+    if (arith_f64(prec_))
         run<double>(in, in_dist, out, out_dist, batch, inverse, stream);
     else
         run<float>(in, in_dist, out, out_dist, batch, inverse, stream);
@@ -876,15 +883,15 @@ std::vector<PlanHints> plan_candidates(size_t n, int kind, int prec) {
         h.fixed_variant = v;
         out.push_back(h);
     }
-    if (prec >= kF16) {
+    if (stored_16(prec)) {
         PlanHints h;
         h.mfma = 1;
         out.push_back(h);
     }
-    const int maxr = prec == kF64 ? 16 : 25;
+    const int maxr = arith_f64(prec) ? 16 : 25;
     if (cn <= (1u << 20)) {
         for (int ept : {kEptSmall, kEptLarge}) {
-            if (prec == kF64 && ept > kEptSmall) continue;
+            if (arith_f64(prec) && ept > kEptSmall) continue;
             std::vector<std::pair<double, std::vector<int>>> all;
             std::vector<int> cur;
             collect_radices((int)cn, maxr, cur, (int)cn, ept, max_block(prec), all);

@@ -19,10 +19,18 @@ namespace sfft {
 
 enum Kind : int { kC2C = 0, kR2C = 1, kR2CHalf = 2 };
 // kF16 / kBF16: float arithmetic, 16-bit HBM storage (complex-to-complex only).
-enum Prec : int { kF32 = 0, kF64 = 1, kF16 = 2, kBF16 = 3 };
+// kF32X: FP32 HBM storage (the FFT<float> layout), FP64 twiddles, LDS and
+// arithmetic (complex-to-complex only).
+enum Prec : int { kF32 = 0, kF64 = 1, kF16 = 2, kBF16 = 3, kF32X = 4 };
+
+inline bool stored_16(int prec) { return prec == kF16 || prec == kBF16; }
+// The arithmetic (registers, LDS, twiddles) is FP64.
+inline bool arith_f64(int prec) { return prec == kF64 || prec == kF32X; }
+// HBM storage differs from the arithmetic type.
+inline bool mixed_storage(int prec) { return stored_16(prec) || prec == kF32X; }
 
 // Bytes of one real component in HBM.
-inline size_t real_bytes(int prec) { return prec == kF64 ? 8 : (prec >= kF16 ? 2 : 4); }
+inline size_t real_bytes(int prec) { return prec == kF64 ? 8 : (stored_16(prec) ? 2 : 4); }
 
 // Error with an ABI status code.
 struct Error : std::runtime_error {

@@ -34,6 +34,11 @@ constexpr int kMaxBlock16 = 512;
 hipError_t launch_pass_16(const PassArgs<float> &a, int storage, int ept, unsigned grid,
                           unsigned block, size_t lds_bytes, hipStream_t stream);
 
+// FP32 HBM storage with FP64 arithmetic (f32x.hip); a.in / a.out address
+// 8-byte complex values, the descriptor's tables are FP64.
+hipError_t launch_pass_f32x(const PassArgs<double> &a, unsigned grid, unsigned block,
+                            size_t lds_bytes, hipStream_t stream);
+
 // In-place forward real post-process of batch spectra of length h (distance
 // dist complex values) after a two-pass complex FFT.
 template <typename T>

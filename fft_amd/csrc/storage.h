@@ -2,8 +2,9 @@
 //
 // The arithmetic type of a transform is T (float or double).  Its HBM data
 // may be stored in T itself or, for T = float, in 16-bit formats (IEEE half
-// or bfloat16, interleaved (re, im), 4 bytes per complex value): loads widen
-// to float, stores round to nearest even.  LDS and registers always hold T.
+// or bfloat16, interleaved (re, im), 4 bytes per complex value), or, for
+// T = double, in FP32 (8 bytes per complex value): loads widen, stores round
+// to nearest even.  LDS and registers always hold T.
 //
 // With 16-bit storage the radix-16 butterflies run on the matrix cores: a
 // 16-point DFT of 32 columns is the real 32x32 product
@@ -34,6 +35,7 @@ namespace sfft {
 struct StoreNative {};
 struct StoreF16 {};
 struct StoreBF16 {};
+struct StoreF32 {};  // FP32 complex values in HBM under FP64 arithmetic
 
 // float -> bfloat16 bits, round to nearest even (NaN stays NaN).
 __device__ __forceinline__ uint32_t f32_to_bf16(float x) {
@@ -84,6 +86,20 @@ struct Storage<float, StoreBF16> {
         typedef __bf16 b2 __attribute__((ext_vector_type(2)));
         const b2 w = {(__bf16)v.x, (__bf16)v.y};  // one packed round-to-nearest-even convert
         static_cast<uint32_t *>(p)[i] = __builtin_bit_cast(uint32_t, w);
+    }
+};
+
+// FP32 storage of an FP64 transform: loads widen exactly, stores round to
+// nearest even once.
+template <>
+struct Storage<double, StoreF32> {
+    static constexpr bool native = false;
+    __device__ static C2<double> load(const void *p, int64_t i) {
+        const C2<float> v = static_cast<const C2<float> *>(p)[i];
+        return {(double)v.x, (double)v.y};
+    }
+    __device__ static void store(void *p, int64_t i, C2<double> v) {
+        static_cast<C2<float> *>(p)[i] = {(float)v.x, (float)v.y};
     }
 };
 

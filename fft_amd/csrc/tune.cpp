@@ -87,7 +87,7 @@ TuneResult autotune(size_t n, int kind, int prec, size_t batch, int reps) {
     Stream st;
     Events ev;
     const int f64 = prec == kF64;
-    if (prec >= kF16)  // 16-bit values: zeros (the timing does not depend on them)
+    if (stored_16(prec))  // 16-bit values: zeros (the timing does not depend on them)
         check_hip(hipMemsetAsync(x.p, 0, in_b, st.s), "clear");
     else
         check_hip(fill_synthetic(x.p, in_b / (f64 ? 8 : 4), f64, 0, 0x5EEDull, st.s), "fill");
@@ -293,7 +293,7 @@ size_t tune_load(const std::string &path, bool check_device) {
         if (e.type != Node::kObj) throw Error(SFFT_ERR_INVALID_ARG, "tuning file: bad entry");
         const long n = num_field(e, "n", 0);
         const long kind = num_field(e, "kind", -1), prec = num_field(e, "precision", -1);
-        if (n < 1 || kind < 0 || kind > 2 || prec < 0 || prec > 3)
+        if (n < 1 || kind < 0 || kind > 2 || prec < 0 || prec > kF32X)
             throw Error(SFFT_ERR_INVALID_ARG, "tuning file: bad key");
         TuneResult t;
         t.hints.fixed_variant = (int)num_field(e, "variant", -1);

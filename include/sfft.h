@@ -63,6 +63,15 @@ extern "C" {
  * by default they run as float butterflies (faster on MI355X). */
 #define SFFT_F16 2
 #define SFFT_BF16 3
+/* FP32 HBM storage (complex64: the same buffers as FFT<float>) with FP64
+ * twiddles, LDS and arithmetic, complex-to-complex only.  The reference's
+ * FFT<float> computes its twiddle phases and products in float
+ * (signalsmith-fft.h:123-124, :198-200, :217-286); here every twiddle
+ * product and butterfly is formed in FP64, so the only FP32 roundings are
+ * the stored output (and the stored intermediate of two-pass lengths):
+ * relative L2 error about 4e-8 against FFT<double>, where F32 sits at
+ * 1.2e-7 to 1.9e-7.  Costs FP64 registers and LDS (a smaller one-pass range). */
+#define SFFT_F32X 4
 
 /* directions */
 #define SFFT_FORWARD 0 /* fft()  */
