@@ -91,6 +91,19 @@ class Plan:
         L.check(L.lib().sfft_execute(self._h, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
                                      batch, direction, ctypes.c_void_p(stream_handle)))
 
+    def workspace_size(self, batch: int, in_place: bool) -> int:
+        """Device workspace bytes an execute call needs (sfft_plan_workspace_size)."""
+        b = ctypes.c_size_t(0)
+        L.check(L.lib().sfft_plan_workspace_size(self._h, batch, int(in_place), ctypes.byref(b)))
+        return b.value
+
+    def execute_workspace_ptr(self, in_ptr: int, out_ptr: int, batch: int, direction: int,
+                              ws_ptr: int, ws_bytes: int, stream_handle: int) -> None:
+        """sfft_execute_workspace: the caller's device workspace, no allocation."""
+        L.check(L.lib().sfft_execute_workspace(
+            self._h, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr), batch, direction,
+            ctypes.c_void_p(ws_ptr), ws_bytes, ctypes.c_void_p(stream_handle)))
+
 
 def _check_device_pair(inp: torch.Tensor, out: torch.Tensor):
     if not (inp.is_cuda and out.is_cuda):

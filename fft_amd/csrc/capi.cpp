@@ -196,10 +196,10 @@ int sfft_plan_get_info(sfft_plan plan, sfft_plan_info *info) {
         for (int s = 0; s < pd.nstages && s < 16; ++s) info->radices[s] = pd.st[s].radix;
         info->threads_per_transform = pd.tpt;
         info->transforms_per_block = pd.tpw;
-        info->workspace_bytes = 0;
+        info->workspace_bytes = p.workspace_bytes(1, true);
         info->table_bytes = p.table_bytes();
         info->mfma_stages = p.mfma_stages();
-        info->in_place = p.in_place() ? 1 : 0;
+        info->in_place = 1;
         return SFFT_OK;
     });
 }
@@ -213,6 +213,31 @@ int sfft_execute(sfft_plan plan, const void *in, void *out, size_t batch, int di
         const sfft::Plan &p = *plan->plan;
         p.execute(in, p.default_in_dist(direction), out, p.default_out_dist(direction), batch,
                   direction, (hipStream_t)stream);
+        return SFFT_OK;
+    });
+}
+
+int sfft_plan_workspace_size(sfft_plan plan, size_t batch, int in_place, size_t *bytes) {
+    return guarded([&] {
+        if (!plan || !bytes) return fail(SFFT_ERR_INVALID_ARG, "NULL argument");
+        *bytes = plan->plan->workspace_bytes(batch, in_place != 0);
+        return SFFT_OK;
+    });
+}
+
+int sfft_execute_workspace(sfft_plan plan, const void *in, void *out, size_t batch,
+                           int direction, void *workspace, size_t workspace_bytes,
+                           void *stream) {
+    return guarded([&] {
+        if (!plan) return fail(SFFT_ERR_INVALID_ARG, "plan is NULL");
+        if (direction != SFFT_FORWARD && direction != SFFT_INVERSE)
+            return fail(SFFT_ERR_INVALID_ARG, "direction must be SFFT_FORWARD or SFFT_INVERSE");
+        const sfft::Plan &p = *plan->plan;
+        const bool in_place = in != nullptr && in == out;
+        if (p.workspace_bytes(batch, in_place) && !workspace)
+            return fail(SFFT_ERR_INVALID_ARG, "this call needs a workspace");
+        p.execute(in, p.default_in_dist(direction), out, p.default_out_dist(direction), batch,
+                  direction, (hipStream_t)stream, workspace, workspace_bytes);
         return SFFT_OK;
     });
 }

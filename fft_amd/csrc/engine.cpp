@@ -474,14 +474,8 @@ void Plan::build_bluestein() {
 
 template <typename T>
 void Plan::run_bluestein(const void *in, int64_t din, void *out, int64_t dout, size_t batch,
-                         int inverse, hipStream_t stream) const {
+                         int inverse, hipStream_t stream, void *ws) const {
     const size_t row = bs_m_ * sizeof(C2<T>);
-    if (batch > (0x7fffffffull / bs_m_)) throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
-    void *ws = nullptr;
-    if (hipMallocAsync(&ws, 2 * row * batch, stream) != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(SFFT_ERR_OUT_OF_MEMORY, "chirp-z workspace allocation failed");
-    }
     C2<T> *a = (C2<T> *)ws, *c = (C2<T> *)((char *)ws + row * batch);
     const bool real = kind_ != kC2C;
     const int mode = !real ? kModeC2C : (inverse ? kModeC2RLoad : kModeR2CLoad);
@@ -497,7 +491,6 @@ void Plan::run_bluestein(const void *in, int64_t din, void *out, int64_t dout, s
                          inverse, (real && inverse && hs) ? (const C2<T> *)d_rot_ : nullptr,
                          stream),
               "chirp-z post-multiply");
-    (void)hipFreeAsync(ws, stream);
     if (real && !inverse)
         check_hip(launch_real_post<T>((C2<T> *)out, dout, (int)cn_, batch,
                                       (const C2<T> *)d_real_tw_, hs, stream),
@@ -635,7 +628,7 @@ size_t Plan::slot_lds_bytes(size_t p) const {
 
 template <typename T>
 void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
-               int inverse, hipStream_t stream) const {
+               int inverse, hipStream_t stream, void *ws) const {
     if (passes_.size() == 1 && passes_[0].fixed && in_dist == default_in_dist(inverse) && out_dist == default_out_dist(inverse)) {
         run_fixed<T>(in, out, batch, inverse, stream);
         return;
@@ -652,12 +645,21 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         }
     }
     if (bs_inner_) {
-        run_bluestein<T>(in, din, out, dout, batch, inverse, stream);
+        run_bluestein<T>(in, din, out, dout, batch, inverse, stream, ws);
         return;
     }
     const size_t np = passes_.size();
+    // In place on a two-pass length the first pass writes its intermediate to
+    // the workspace (cn values per transform) and the second pass reads it
+    // there; out of place the output buffer holds the intermediate.
+    const int64_t dws = (int64_t)cn_;
     for (size_t p = 0; p < np; ++p) {
         const PassPlan &pp = passes_[p];
+        const bool last = p + 1 == np;
+        const void *src = p == 0 ? in : (ws ? ws : out);
+        const int64_t dsrc = p == 0 ? din : (ws ? dws : dout);
+        void *dst = (!last && ws) ? ws : out;
+        const int64_t ddst = (!last && ws) ? dws : dout;
         const size_t slots = batch * (size_t)pp.slots_per_transform;
         if (slots > 0x7fffffffull) throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
         if (pp.fixed && pp.fixed->real == 2) {
@@ -668,15 +670,15 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
             const bool plain = m.mode == kModeC2C ||
                                ((m.mode == kModeR2CLoad || m.mode == kModeC2RStore) && !m.half_shift);
             if (plain) {
-                run_fixed_columns<T>(p, in, din, out, dout, batch, inverse, stream);
+                run_fixed_columns<T>(p, src, dsrc, dst, ddst, batch, inverse, stream);
                 continue;
             }
         }
         PassArgs<T> a = pass_args<T>(p, inverse);
-        a.iin.batch_stride = pp.in_from_out ? dout : din;
-        a.iout.batch_stride = dout;
-        a.in = pp.in_from_out ? (const C2<T> *)out : (const C2<T> *)in;
-        a.out = (C2<T> *)out;
+        a.iin.batch_stride = dsrc;
+        a.iout.batch_stride = ddst;
+        a.in = (const C2<T> *)src;
+        a.out = (C2<T> *)dst;
         a.total_slots = (int32_t)slots;
         const unsigned grid = (unsigned)((slots + pp.pd.tpw - 1) / pp.pd.tpw);
         if constexpr (std::is_same<T, float>::value) {
@@ -738,13 +740,13 @@ template void Plan::run_fixed<float>(const void *, void *, size_t, int, hipStrea
 // kernel addresses HBM with 32-bit byte offsets from the launch base, so the
 // batch is split into launches spanning less than 4 GiB.
 template <typename T>
-void Plan::run_fixed_columns(size_t p, const void *in, int64_t din, void *out, int64_t dout,
+void Plan::run_fixed_columns(size_t p, const void *src, int64_t dsrc, void *dst, int64_t ddst,
                              size_t batch, int inverse, hipStream_t stream) const {
     const PassPlan &pp = passes_[p];
     const FixedKernel &fk = *pp.fixed;
     const int mode = p == 0 ? kModeColRow : kModeColCol;
     const size_t eb = 2 * real_bytes(prec_);  // bytes per complex value in HBM
-    const int64_t dmax = std::max(pp.in_from_out ? dout : din, dout);
+    const int64_t dmax = std::max(dsrc, ddst);
     const size_t chunk = std::max<size_t>(1, ((1ull << 32) - 1) / ((size_t)dmax * eb));
     if ((size_t)pp.slots_per_transform * std::min(chunk, batch) > 0x7fffffffull)
         throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
@@ -755,19 +757,17 @@ void Plan::run_fixed_columns(size_t p, const void *in, int64_t din, void *out, i
     for (size_t c0 = 0; c0 < batch; c0 += chunk) {
         const size_t cb = std::min(chunk, batch - c0);
         FixedArgs<T> a{};
-        const char *src = pp.in_from_out ? (const char *)out : (const char *)in;
-        const int64_t ds = pp.in_from_out ? dout : din;
-        a.in = (const C2<T> *)(src + (int64_t)c0 * ds * (int64_t)eb);
-        a.out = (C2<T> *)((char *)out + (int64_t)c0 * dout * (int64_t)eb);
+        a.in = (const C2<T> *)((const char *)src + (int64_t)c0 * dsrc * (int64_t)eb);
+        a.out = (C2<T> *)((char *)dst + (int64_t)c0 * ddst * (int64_t)eb);
         a.tw = (const C2<T> *)d_tw_ + tw0;
         a.total_slots = (int32_t)(cb * (size_t)pp.slots_per_transform);
         a.flip_in = conj.conj_in ? flip : 0u;
         a.flip_out = conj.conj_out ? flip : 0u;
         a.spt = make_fastdiv((uint32_t)pp.slots_per_transform);
-        a.in_bs = (int32_t)ds;
+        a.in_bs = (int32_t)dsrc;
         a.in_ss = (int32_t)pp.in_slot_stride;
         a.in_es = (int32_t)pp.in_elem_stride;
-        a.out_bs = (int32_t)dout;
+        a.out_bs = (int32_t)ddst;
         a.out_ss = (int32_t)pp.out_slot_stride;
         a.out_es = (int32_t)pp.out_elem_stride;
         a.post_tw = pp.post_tw;
@@ -785,8 +785,15 @@ template void Plan::run_fixed_columns<double>(size_t, const void *, int64_t, voi
                                               size_t, int, hipStream_t) const;
 template void Plan::run_fixed<double>(const void *, void *, size_t, int, hipStream_t) const;
 
+size_t Plan::workspace_bytes(size_t batch, bool in_place) const {
+    const size_t cb = 2 * real_bytes(prec_);  // one complex value in HBM
+    if (bs_inner_) return 2 * bs_m_ * cb * batch;
+    if (in_place && passes_.size() > 1) return cn_ * cb * batch;
+    return 0;
+}
+
 void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
-                   int inverse, hipStream_t stream) const {
+                   int inverse, hipStream_t stream, void *ws, size_t ws_bytes) const {
     if (batch == 0) return;
     if (!in || !out) throw Error(SFFT_ERR_INVALID_ARG, "null buffer");
     // overlap check on the byte ranges actually touched
@@ -796,22 +803,47 @@ void Plan::execute(const void *in, size_t in_dist, void *out, size_t out_dist, s
     const size_t in_len = ((batch - 1) * in_dist + default_in_dist(inverse)) * in_unit;
     const size_t out_len = ((batch - 1) * out_dist + default_out_dist(inverse)) * out_unit;
     const uintptr_t i0 = (uintptr_t)in, o0 = (uintptr_t)out;
-    if (i0 == o0 && in_dist * in_unit == out_dist * out_unit) {
-        // in place, same layout: every transform is read whole before it is
-        // written (single pass: one workgroup per slot, reads before the
-        // first barrier; chirp-z: the input is consumed into the workspace)
-        if (!in_place())
-            throw Error(SFFT_ERR_ALIASING,
-                        "in-place needs a one-pass or chirp-z plan (this length runs two passes)");
-    } else if (i0 < o0 + out_len && o0 < i0 + in_len) {
+    // in place, same layout: one-pass plans read every transform whole before
+    // writing it (one workgroup per slot, reads before the first barrier),
+    // chirp-z plans consume the input into their workspace, and two-pass
+    // plans keep their intermediate in a workspace
+    const bool in_place = i0 == o0 && in_dist * in_unit == out_dist * out_unit;
+    if (!in_place && i0 < o0 + out_len && o0 < i0 + in_len)
         throw Error(SFFT_ERR_ALIASING, "input and output overlap other than exactly in place");
-    }
     if (in_dist < default_in_dist(inverse) || out_dist < default_out_dist(inverse))
         throw Error(SFFT_ERR_INVALID_ARG, "batch distance shorter than one transform");
+    if (bs_inner_ && batch > (0x7fffffffull / bs_m_))
+        throw Error(SFFT_ERR_INVALID_ARG, "batch too large");
+    const size_t need = workspace_bytes(batch, in_place);
+    struct Owned {
+        void *p = nullptr;
+        hipStream_t s = nullptr;
+        ~Owned() {
+            if (p) (void)hipFreeAsync(p, s);
+        }
+    } owned;
+    if (need && ws) {
+        if (ws_bytes < need)
+            throw Error(SFFT_ERR_INVALID_ARG, "workspace of " + std::to_string(ws_bytes) +
+                                                  " bytes, the call needs " + std::to_string(need));
+        const uintptr_t w0 = (uintptr_t)ws;
+        if ((w0 < i0 + in_len && i0 < w0 + need) || (w0 < o0 + out_len && o0 < w0 + need))
+            throw Error(SFFT_ERR_ALIASING, "workspace overlaps the input or the output");
+    } else if (need) {
+        if (hipMallocAsync(&owned.p, need, stream) != hipSuccess) {
+            (void)hipGetLastError();
+            owned.p = nullptr;
+            throw Error(SFFT_ERR_OUT_OF_MEMORY, "workspace allocation failed");
+        }
+        owned.s = stream;
+        ws = owned.p;
+    } else {
+        ws = nullptr;
+    }
     if (arith_f64(prec_))
-        run<double>(in, in_dist, out, out_dist, batch, inverse, stream);
+        run<double>(in, in_dist, out, out_dist, batch, inverse, stream, ws);
     else
-        run<float>(in, in_dist, out, out_dist, batch, inverse, stream);
+        run<float>(in, in_dist, out, out_dist, batch, inverse, stream, ws);
 }
 
 static std::mutex g_cache_mu;

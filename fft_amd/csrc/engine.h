@@ -93,8 +93,14 @@ class Plan {
     const std::vector<PassPlan> &passes() const { return passes_; }
     size_t table_bytes() const { return table_bytes_; }
     int mfma_stages() const;
-    // Whether in == out (same layout) is allowed: one-pass and chirp-z plans.
-    bool in_place() const { return bs_inner_ != nullptr || passes_.size() == 1; }
+    // in == out (same layout) runs without a workspace: one-pass and chirp-z
+    // plans read each transform whole before writing it; two-pass plans run
+    // in place through a workspace for the intermediate.
+    bool in_place_direct() const { return bs_inner_ != nullptr || passes_.size() == 1; }
+    // Device workspace an execute call of `batch` transforms needs: chirp-z
+    // rows (2 M values per transform) or, in place on a two-pass length, the
+    // intermediate (cn values per transform); 0 if none.
+    size_t workspace_bytes(size_t batch, bool in_place) const;
     // Chirp-z plan: the length has a prime factor the kernels do not plan;
     // the transform runs as a convolution through FFTs of length bluestein_m().
     bool bluestein() const { return bs_inner_ != nullptr; }
@@ -102,9 +108,12 @@ class Plan {
     const Plan *bluestein_inner() const { return bs_inner_.get(); }
 
     // in/out: device pointers; dists in elements of the in/out type
-    // (complex for C2C, real/complex for the real kinds).
+    // (complex for C2C, real/complex for the real kinds).  ws / ws_bytes: a
+    // caller's device workspace of at least workspace_bytes(batch, in == out)
+    // bytes; without one, a needed workspace is allocated stream-ordered
+    // (hipMallocAsync) and freed after the call.
     void execute(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch,
-                 int inverse, hipStream_t stream) const;
+                 int inverse, hipStream_t stream, void *ws = nullptr, size_t ws_bytes = 0) const;
 
     // Static launch arguments of pass p for a direction (in/out/total_slots
     // left empty; batch distances = contiguous defaults).
@@ -122,19 +131,21 @@ class Plan {
   private:
     template <typename T>
     void build_tables();
+    // ws: the workspace when one is needed (chirp-z rows, or the
+    // intermediate of a two-pass plan run in place), else null
     template <typename T>
     void run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch, int inverse,
-             hipStream_t stream) const;
+             hipStream_t stream, void *ws) const;
     template <typename T>
     void run_fixed(const void *in, void *out, size_t batch, int inverse, hipStream_t stream) const;
     template <typename T>
-    void run_fixed_columns(size_t p, const void *in, int64_t din, void *out, int64_t dout,
+    void run_fixed_columns(size_t p, const void *src, int64_t dsrc, void *dst, int64_t ddst,
                            size_t batch, int inverse, hipStream_t stream) const;
     template <typename T>
     void build_bluestein();
     template <typename T>
     void run_bluestein(const void *in, int64_t din, void *out, int64_t dout, size_t batch,
-                       int inverse, hipStream_t stream) const;
+                       int inverse, hipStream_t stream, void *ws) const;
 
     size_t n_;
     int kind_, prec_;

@@ -86,12 +86,14 @@ typedef struct {
     int nradices;           /* radices of the first pass, in stage order */
     int radices[16];
     int threads_per_transform, transforms_per_block;
-    size_t workspace_bytes; /* device scratch held by the plan (0: none) */
+    size_t workspace_bytes; /* device workspace per transform of an in-place call
+                               (chirp-z lengths: every call); 0: none */
     size_t table_bytes;     /* device twiddle tables held by the plan */
     int mfma_stages;        /* radix-16 stages run as MFMA contractions (16-bit storage) */
     size_t chirp_length;    /* lengths with prime factors > 13: the plan is a chirp-z
                                convolution through FFTs of this length (0: direct) */
-    int in_place;           /* 1: in == out (same layout) is allowed */
+    int in_place;           /* 1: in == out (same layout) is allowed (every plan:
+                               two-pass lengths run it through the workspace) */
 } sfft_plan_info;
 
 /* ---- library ---- */
@@ -181,9 +183,24 @@ int sfft_tune_clear(void);
  *   R2C fwd:   in: batch x n reals;  out: batch x n/2 complex,
  *              out[0] = (DC, Nyquist) packed (not for the half-shift kind)
  *   R2C inv:   in: batch x n/2 complex; out: batch x n reals
- * batch == 0 is a no-op.  The call is asynchronous on `stream`. */
+ * batch == 0 is a no-op.  The call is asynchronous on `stream`.
+ * in == out transforms in place (the reference accepts any iterator pair,
+ * signalsmith-fft.h:55-67, but corrupts in place); other overlaps return
+ * SFFT_ERR_ALIASING.  Chirp-z lengths, and two-pass lengths in place, need
+ * a device workspace (sfft_plan_workspace_size): sfft_execute allocates it
+ * stream-ordered (hipMallocAsync, released after the call). */
 int sfft_execute(sfft_plan plan, const void *in, void *out, size_t batch, int direction,
                  void *stream);
+
+/* Bytes of device workspace an execute call of `batch` transforms needs
+ * (in_place: in == out); 0 if none. */
+int sfft_plan_workspace_size(sfft_plan plan, size_t batch, int in_place, size_t *bytes);
+/* sfft_execute with a caller-owned device workspace of at least
+ * sfft_plan_workspace_size bytes (may be NULL when that is 0): the call
+ * allocates nothing.  Calls that share a workspace must not overlap. */
+int sfft_execute_workspace(sfft_plan plan, const void *in, void *out, size_t batch,
+                           int direction, void *workspace, size_t workspace_bytes,
+                           void *stream);
 
 /* Same as sfft_execute with an explicit distance (in elements of the
  * input / output type) between consecutive transforms of the batch. */

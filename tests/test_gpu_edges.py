@@ -209,10 +209,59 @@ def test_in_place_one_pass_and_chirp_z(n, kind, prec):
         assert torch.equal(buf.view(torch.uint8).flatten(), want), (n, kind, d)
 
 
-def test_in_place_two_pass_rejected():
+@pytest.mark.parametrize("n,kind,prec,batch", [
+    (1 << 16, L.C2C, L.F32, 3), (1 << 20, L.C2C, L.F32, 2), (3 << 15, L.C2C, L.F64, 2),
+    (1 << 17, L.R2C, L.F32, 2), (3 << 16, L.R2C_HALFSHIFT, L.F32, 2), (1 << 16, L.C2C, L.F32X, 2)])
+def test_in_place_two_pass(n, kind, prec, batch):
+    """Two-pass lengths in place (in == out) through the workspace: bitwise
+    equal to out of place, with the library's workspace and with the caller's."""
+    p = fft_amd.Plan(n, kind, prec)
+    info = p.info()
+    assert info["passes"] == 2 and info["in_place"] == 1
+    cb = 16 if prec == L.F64 else 8
+    assert info["workspace_bytes"] == (n if kind == L.C2C else n // 2) * cb
+    assert p.workspace_size(batch, True) == batch * info["workspace_bytes"]
+    assert p.workspace_size(batch, False) == 0
+    cdt = torch.complex128 if prec == L.F64 else torch.complex64
+    rdt = torch.float64 if prec == L.F64 else torch.float32
+    sh = torch.cuda.current_stream().cuda_stream
+    for d in (0, 1):
+        if kind == L.C2C or d == 1:
+            x = torch.randn(batch, n if kind == L.C2C else n // 2, dtype=cdt, device="cuda")
+        else:
+            x = torch.randn(batch, n, dtype=rdt, device="cuda")
+        nbytes = x.numel() * x.element_size()
+        want = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        p.execute_ptr(x.data_ptr(), want.data_ptr(), batch, d, sh)
+        buf = x.clone()
+        p.execute_ptr(buf.data_ptr(), buf.data_ptr(), batch, d, sh)
+        buf2 = x.clone()
+        ws = torch.empty(p.workspace_size(batch, True), dtype=torch.uint8, device="cuda")
+        p.execute_workspace_ptr(buf2.data_ptr(), buf2.data_ptr(), batch, d, ws.data_ptr(),
+                                ws.numel(), sh)
+        torch.cuda.synchronize()
+        assert torch.equal(buf.view(torch.uint8).flatten(), want), (n, kind, d)
+        assert torch.equal(buf2.view(torch.uint8).flatten(), want), (n, kind, d)
+
+
+def test_workspace_errors():
     p = fft_amd.Plan(1 << 16, L.C2C, L.F32)
-    assert p.info()["in_place"] == 0
     x = torch.zeros(2, 1 << 16, dtype=torch.complex64, device="cuda")
-    rc = L.lib().sfft_execute(p._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(x.data_ptr()),
-                              2, 0, None)
+    lib = L.lib()
+    ws = torch.empty(p.workspace_size(2, True), dtype=torch.uint8, device="cuda")
+    xp = ctypes.c_void_p(x.data_ptr())
+    # in place needs the workspace; one byte short is refused
+    rc = lib.sfft_execute_workspace(p._h, xp, xp, 2, 0, None, 0, None)
+    assert rc == L.SFFT_ERR_INVALID_ARG
+    rc = lib.sfft_execute_workspace(p._h, xp, xp, 2, 0, ctypes.c_void_p(ws.data_ptr()),
+                                    ws.numel() - 1, None)
+    assert rc == L.SFFT_ERR_INVALID_ARG
+    # a workspace inside the data is refused
+    rc = lib.sfft_execute_workspace(p._h, xp, xp, 1, 0, ctypes.c_void_p(x.data_ptr()),
+                                    ws.numel(), None)
     assert rc == L.SFFT_ERR_ALIASING
+    # out of place needs none
+    y = torch.empty_like(x)
+    rc = lib.sfft_execute_workspace(p._h, xp, ctypes.c_void_p(y.data_ptr()), 2, 0, None, 0, None)
+    assert rc == L.SFFT_OK
+    torch.cuda.synchronize()
