@@ -40,7 +40,9 @@ EXPORTED = [
     "sfft_server_drain", "sfft_server_fetch", "sfft_server_release", "sfft_server_now",
     "sfft_server_run_poisson",
     "sfft_malloc_device", "sfft_free_device", "sfft_malloc_host", "sfft_free_host",
-    "sfft_memcpy", "sfft_stream_synchronize",
+    "sfft_memcpy", "sfft_stream_synchronize", "sfft_host_device_pointer",
+    "sfft_stream_create", "sfft_stream_destroy", "sfft_event_create", "sfft_event_destroy",
+    "sfft_event_record", "sfft_event_synchronize",
     "sfft_fill",
 ]
 

@@ -309,7 +309,10 @@ int sfft_malloc_host(void **ptr, size_t bytes) {
     return guarded([&] {
         if (!ptr) return fail(SFFT_ERR_INVALID_ARG, "NULL pointer");
         *ptr = nullptr;
-        if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        // mapped for device access, coherent: kernels that read and write it
+        // directly see the host's latest values on every launch
+        if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess) {
             (void)hipGetLastError();
             return fail(SFFT_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
         }
@@ -341,6 +344,66 @@ int sfft_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream
 int sfft_stream_synchronize(void *stream) {
     return guarded([&] {
         sfft::check_hip(hipStreamSynchronize((hipStream_t)stream), "stream synchronize");
+        return SFFT_OK;
+    });
+}
+
+int sfft_host_device_pointer(void **device_ptr, void *host_ptr) {
+    return guarded([&] {
+        if (!device_ptr || !host_ptr) return fail(SFFT_ERR_INVALID_ARG, "NULL pointer");
+        *device_ptr = nullptr;
+        sfft::check_hip(hipHostGetDevicePointer(device_ptr, host_ptr, 0),
+                        "device address of pinned host memory");
+        return SFFT_OK;
+    });
+}
+
+int sfft_stream_create(void **stream) {
+    return guarded([&] {
+        if (!stream) return fail(SFFT_ERR_INVALID_ARG, "NULL pointer");
+        hipStream_t s = nullptr;
+        sfft::check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream create");
+        *stream = s;
+        return SFFT_OK;
+    });
+}
+
+int sfft_stream_destroy(void *stream) {
+    return guarded([&] {
+        if (stream) sfft::check_hip(hipStreamDestroy((hipStream_t)stream), "stream destroy");
+        return SFFT_OK;
+    });
+}
+
+int sfft_event_create(void **event) {
+    return guarded([&] {
+        if (!event) return fail(SFFT_ERR_INVALID_ARG, "NULL pointer");
+        hipEvent_t e = nullptr;
+        sfft::check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
+        *event = e;
+        return SFFT_OK;
+    });
+}
+
+int sfft_event_destroy(void *event) {
+    return guarded([&] {
+        if (event) sfft::check_hip(hipEventDestroy((hipEvent_t)event), "event destroy");
+        return SFFT_OK;
+    });
+}
+
+int sfft_event_record(void *event, void *stream) {
+    return guarded([&] {
+        if (!event) return fail(SFFT_ERR_INVALID_ARG, "NULL event");
+        sfft::check_hip(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "event record");
+        return SFFT_OK;
+    });
+}
+
+int sfft_event_synchronize(void *event) {
+    return guarded([&] {
+        if (!event) return fail(SFFT_ERR_INVALID_ARG, "NULL event");
+        sfft::check_hip(hipEventSynchronize((hipEvent_t)event), "event synchronize");
         return SFFT_OK;
     });
 }

@@ -305,11 +305,23 @@ int sfft_server_run_poisson(sfft_server server, const sfft_poisson_config *confi
 
 int sfft_malloc_device(void **ptr, size_t bytes);
 int sfft_free_device(void *ptr);
-/* page-locked host memory (staging for asynchronous copies) */
+/* page-locked host memory (staging for asynchronous copies), mapped for
+ * device access and coherent */
 int sfft_malloc_host(void **ptr, size_t bytes);
 int sfft_free_host(void *ptr);
 int sfft_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
 int sfft_stream_synchronize(void *stream);
+/* Device address of page-locked host memory from sfft_malloc_host: kernels
+ * may read and write it directly (over the host link). */
+int sfft_host_device_pointer(void **device_ptr, void *host_ptr);
+/* Streams (non-blocking, default priority) and events, so that bindings can
+ * overlap staging copies with host work. */
+int sfft_stream_create(void **stream);
+int sfft_stream_destroy(void *stream);
+int sfft_event_create(void **event);
+int sfft_event_destroy(void *event);
+int sfft_event_record(void *event, void *stream);
+int sfft_event_synchronize(void *event);
 
 /* ---- synthetic inputs (seeded, counter-based splitmix64) ---- */
 

@@ -18,6 +18,7 @@
 using cd = std::complex<double>;
 
 static int failures = 0;
+static const int FFTOptions_halfFreqShift = signalsmith::FFTOptions::halfFreqShift;
 #define CHECK(cond, ...)                                   \
     do {                                                   \
         if (!(cond)) {                                     \
@@ -210,6 +211,128 @@ static void float_device_batch() {  // device pointers, a batch, FFT<float>
     sfft_free_device(dback);
 }
 
+// Host staging, both routes (the kernel on the page-locked buffers for small
+// one-pass lengths, chunked copies for large ones), against the device-pointer
+// path on the same values: bitwise equal.
+template <typename V>
+static void staged_matches_device(size_t n) {
+    using C = std::complex<V>;
+    signalsmith::FFT<V> fft(n);
+    std::vector<C> in(n), out(n), back(n), dev_out(n);
+    for (auto &v : in) v = C((V)uniform_half(), (V)uniform_half());
+    fft.fft(in, out);
+    void *din = nullptr, *dout = nullptr;
+    const size_t bytes = n * sizeof(C);
+    sfft_malloc_device(&din, bytes);
+    sfft_malloc_device(&dout, bytes);
+    sfft_memcpy(din, in.data(), bytes, SFFT_COPY_H2D, nullptr);
+    fft.fftDevice((const C *)din, (C *)dout);
+    sfft_memcpy(dev_out.data(), dout, bytes, SFFT_COPY_D2H, nullptr);
+    sfft_stream_synchronize(nullptr);
+    bool same = true;
+    for (size_t i = 0; i < n; ++i) same = same && out[i] == dev_out[i];
+    CHECK(same, "host-staged and device results differ, N=%zu", n);
+    // in place on the host vector, and the inverse back
+    std::vector<C> inplace = in;
+    fft.fft(inplace, inplace);
+    same = true;
+    for (size_t i = 0; i < n; ++i) same = same && inplace[i] == out[i];
+    CHECK(same, "in-place host transform differs, N=%zu", n);
+    fft.ifft(out, back);
+    double err = 0, ref = 0;
+    for (size_t i = 0; i < n; ++i) {
+        err += std::norm(std::complex<double>(back[i]) / (double)n - std::complex<double>(in[i]));
+        ref += std::norm(std::complex<double>(in[i]));
+    }
+    CHECK(std::sqrt(err / ref) < (sizeof(V) == 4 ? 1e-6 : 1e-13), "round trip N=%zu: %g", n,
+          std::sqrt(err / ref));
+    sfft_free_device(din);
+    sfft_free_device(dout);
+}
+
+template <int flags>
+static void real_staged_matches_device(size_t n) {
+    signalsmith::RealFFT<float, flags> rfft(n);
+    std::vector<float> in(n), back(n);
+    std::vector<std::complex<float>> out(n / 2), dev_out(n / 2);
+    for (auto &v : in) v = (float)uniform_half();
+    rfft.fft(in, out);
+    void *din = nullptr, *dout = nullptr;
+    sfft_malloc_device(&din, n * sizeof(float));
+    sfft_malloc_device(&dout, n * sizeof(float));
+    sfft_memcpy(din, in.data(), n * sizeof(float), SFFT_COPY_H2D, nullptr);
+    rfft.fftDevice((const float *)din, (std::complex<float> *)dout);
+    sfft_memcpy(dev_out.data(), dout, n * sizeof(float), SFFT_COPY_D2H, nullptr);
+    sfft_stream_synchronize(nullptr);
+    bool same = true;
+    for (size_t i = 0; i < n / 2; ++i) same = same && out[i] == dev_out[i];
+    CHECK(same, "real host-staged and device results differ, N=%zu", n);
+    rfft.ifft(out, back);
+    double err = 0, ref = 0;
+    for (size_t i = 0; i < n; ++i) {
+        err += (back[i] / (double)n - in[i]) * (back[i] / (double)n - in[i]);
+        ref += (double)in[i] * in[i];
+    }
+    CHECK(std::sqrt(err / ref) < 1e-6, "real round trip N=%zu: %g", n, std::sqrt(err / ref));
+    sfft_free_device(din);
+    sfft_free_device(dout);
+}
+
+// FFT<long double> runs in FP64: the same values as FFT<double> on the same
+// (double-representable) input, and the single-bin closed form.
+static void long_double() {
+    const int n = 96;
+    signalsmith::FFT<long double> fl(n);
+    signalsmith::FFT<double> fd(n);
+    std::vector<std::complex<long double>> in(n), out(n);
+    std::vector<cd> ind(n), outd(n);
+    for (int i = 0; i < n; ++i) {
+        ind[i] = cd(uniform_half(), uniform_half());
+        in[i] = std::complex<long double>(ind[i]);
+    }
+    fl.fft(in, out);
+    fd.fft(ind, outd);
+    bool same = true;
+    for (int i = 0; i < n; ++i) same = same && cd(out[i]) == outd[i];
+    CHECK(same, "FFT<long double> differs from FFT<double>");
+    const int bin = 5;
+    for (int i = 0; i < n; ++i) {
+        const long double ph = 2 * 3.14159265358979323846264338327950288L * i * bin / n;
+        in[i] = std::complex<long double>(std::cos(ph), std::sin(ph));
+    }
+    fl.fft(in, out);
+    double err = 0;
+    for (int i = 0; i < n; ++i) err += std::norm(cd(out[i]) - cd(i == bin ? n : 0, 0));
+    CHECK(std::sqrt(err) / n < 1e-13, "FFT<long double> single bin: %g", std::sqrt(err) / n);
+    signalsmith::RealFFT<long double> rl(64);
+    std::vector<long double> rin(64);
+    std::vector<std::complex<long double>> rout(32);
+    for (auto &v : rin) v = uniform_half();
+    rl.fft(rin, rout);
+    CHECK(std::abs(cd(rout[0]).real() - [&] {
+              double s = 0;
+              for (auto v : rin) s += (double)v;
+              return s;
+          }()) < 1e-12,
+          "RealFFT<long double> DC");
+}
+
+// A failed setSize leaves the object as it was (strong exception guarantee).
+static void failed_set_size_keeps_state() {
+    signalsmith::FFT<float> f(256);
+    bool threw = false;
+    try {
+        f.setSize((size_t(1) << 28) + 1);
+    } catch (const signalsmith::FFTError &) {
+        threw = true;
+    }
+    CHECK(threw, "oversize setSize throws");
+    CHECK(f.size() == 256, "size kept after a failed setSize");
+    std::vector<std::complex<float>> in(256, std::complex<float>(1, 0)), out(256);
+    f.fft(in, out);
+    CHECK(std::abs(out[0] - std::complex<float>(256, 0)) < 1e-3, "transform after a failed setSize");
+}
+
 int main() {
     individual_bins();
     linearity();
@@ -220,6 +343,14 @@ int main() {
     real_tests<true>();
     return_values();
     float_device_batch();
+    staged_matches_device<float>(4096);        // kernel on the staging buffers
+    staged_matches_device<float>(1 << 17);     // 1 MiB: chunked copies, two passes
+    staged_matches_device<double>(3 << 14);    // chunked copies, one pass
+    staged_matches_device<float>(1009);        // chirp-z
+    real_staged_matches_device<0>(2048);
+    real_staged_matches_device<FFTOptions_halfFreqShift>(1 << 19);
+    long_double();
+    failed_set_size_keeps_state();
     if (failures) {
         std::printf("%d failures\n", failures);
         return 1;
