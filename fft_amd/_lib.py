@@ -34,6 +34,7 @@ EXPORTED = [
     "sfft_plan_create_hinted", "sfft_plan_variants",
     "sfft_autotune", "sfft_tune_save", "sfft_tune_load", "sfft_tune_clear",
     "sfft_execute", "sfft_execute_strided", "sfft_plan_workspace_size", "sfft_execute_workspace",
+    "sfft_execute_noalloc",
     "sfft_batch_create", "sfft_batch_destroy", "sfft_batch_execute",
     "sfft_batch_execute_per_size",
     "sfft_server_create", "sfft_server_destroy", "sfft_server_submit", "sfft_server_poll",
@@ -182,6 +183,7 @@ def lib() -> ctypes.CDLL:
         L.sfft_execute_strided.argtypes = [vp, vp, sz, vp, sz, sz, ip, vp]
         L.sfft_plan_workspace_size.argtypes = [vp, sz, ip, ctypes.POINTER(sz)]
         L.sfft_execute_workspace.argtypes = [vp, vp, vp, sz, ip, vp, sz, vp]
+        L.sfft_execute_noalloc.argtypes = [vp, vp, vp, sz, ip, vp]
         L.sfft_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BatchItem), sz, ip]
         L.sfft_batch_destroy.argtypes = [vp]
         L.sfft_batch_execute.argtypes = [vp, vp, vp, vp]

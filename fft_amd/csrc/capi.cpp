@@ -40,10 +40,6 @@ int guarded(F &&f) {
 
 }  // namespace
 
-struct sfft_plan_s {
-    std::shared_ptr<sfft::Plan> plan;
-};
-
 struct sfft_batch_s {
     std::unique_ptr<sfft::UnevenBatch> impl;
 };

@@ -192,3 +192,8 @@ void check_hip(hipError_t e, const char *what);
 std::string &last_error();
 
 }  // namespace sfft
+
+// The object behind the C ABI's sfft_plan handle.
+struct sfft_plan_s {
+    std::shared_ptr<sfft::Plan> plan;
+};

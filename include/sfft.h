@@ -202,6 +202,18 @@ int sfft_execute_workspace(sfft_plan plan, const void *in, void *out, size_t bat
                            int direction, void *workspace, size_t workspace_bytes,
                            void *stream);
 
+/* sfft_execute that allocates nothing on the device.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The workspace a
+ * call needs (chirp-z lengths; two-pass lengths in place) comes from a
+ * library-owned static device array of SFFT_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends; a call needing more returns SFFT_ERR_BUDGET.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other threads) race. */
+#define SFFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int sfft_execute_noalloc(sfft_plan plan, const void *in, void *out, size_t batch, int direction,
+                         void *stream);
+
 /* Same as sfft_execute with an explicit distance (in elements of the
  * input / output type) between consecutive transforms of the batch. */
 int sfft_execute_strided(sfft_plan plan, const void *in, size_t in_dist, void *out,
