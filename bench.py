@@ -72,7 +72,11 @@ def copy_calibration(torch):
 
 
 class Launch:
-    """One batched launch of the step: a plan, buffers, a direction."""
+    """One batched launch of the step: a plan, buffers, a direction.  Every
+    launch reads a buffer that no launch of the step writes (the inverse legs
+    read a copy of the forward output made at set-up), so with the step's
+    6 GB of traffic between two reads of a buffer no input is still in the
+    256 MB last-level cache: every figure is an HBM measurement."""
 
     def __init__(self, name, plan, x, y, batch, direction, alg_bytes, flops):
         self.name, self.plan, self.x, self.y = name, plan, x, y
@@ -95,11 +99,14 @@ def build_workload(torch, fft_amd, L):
         fft_amd.fill(x, seed=n, dist="normal")
         y = torch.empty_like(x)
         z = torch.empty_like(x)
-        bufs[n] = (x, y, z)
+        plan.execute_ptr(x.data_ptr(), y.data_ptr(), C2C_BATCH, L.FORWARD,
+                         torch.cuda.current_stream().cuda_stream)
+        yi = y.clone()  # the inverse legs' input: spectra, in a buffer the step never writes
+        bufs[n] = (x, y, z, yi)
         ab = 16 * n * C2C_BATCH
         fl = 5 * n * math.log2(n) * C2C_BATCH
         launches.append(Launch(f"c2c_{n}_fwd", plan, x, y, C2C_BATCH, L.FORWARD, ab, fl))
-        launches.append(Launch(f"c2c_{n}_inv", plan, y, z, C2C_BATCH, L.INVERSE, ab, fl))
+        launches.append(Launch(f"c2c_{n}_inv", plan, yi, z, C2C_BATCH, L.INVERSE, ab, fl))
     # RealFFT frames: tones from the seeded stream, noise generated in HBM
     n, frames = REAL_N, REAL_FRAMES
     fr = torch.from_numpy(synth.uniform01(n, frames * 8).reshape(frames, 8) * (n / 2)).cuda()
@@ -115,12 +122,15 @@ def build_workload(torch, fft_amd, L):
     del noise
     yr = torch.empty(frames, n // 2, dtype=torch.complex64, device="cuda")
     zr = torch.empty_like(xr)
-    bufs["real"] = (xr, yr, zr)
     plan = fft_amd.Plan(n, L.R2C, L.F32)
+    plan.execute_ptr(xr.data_ptr(), yr.data_ptr(), frames, L.FORWARD,
+                     torch.cuda.current_stream().cuda_stream)
+    yri = yr.clone()
+    bufs["real"] = (xr, yr, zr, yri)
     ab = 8 * n * frames
     fl = 2.5 * n * math.log2(n) * frames
     launches.append(Launch(f"r2c_{n}_fwd", plan, xr, yr, frames, L.FORWARD, ab, fl))
-    launches.append(Launch(f"c2r_{n}_inv", plan, yr, zr, frames, L.INVERSE, ab, fl))
+    launches.append(Launch(f"c2r_{n}_inv", plan, yri, zr, frames, L.INVERSE, ab, fl))
     torch.cuda.synchronize()
     return launches, bufs
 
@@ -225,11 +235,26 @@ def uneven_batch(torch, fft_amd, count=20000, reps=5):
     return res
 
 
+def facade_round_trip(n=4096, reps=200):
+    """Host-staged fft + ifft round trip through the C++ drop-in header
+    (fft_amd/sfft_facade_latency: std::vector in, std::vector out, wall clock,
+    median of `reps`), run as a child process."""
+    import subprocess
+    exe = os.path.join(ROOT, "fft_amd", "sfft_facade_latency")
+    if not os.path.exists(exe):
+        return {"error": "fft_amd/sfft_facade_latency not built"}
+    r = subprocess.run([exe, str(n), str(reps)], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return {"error": (r.stdout + r.stderr)[-300:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def single_transform(torch, fft_amd, reps=200):
     """One FFT<float> / FFT<double> of N = 4096 plus its ifft round trip, U[-1,1]
     inputs (seed 4096): device-resident latency (HIP events around fft + ifft,
-    median of `reps`) and the host-staged latency of the drop-in path (host
-    array in, host array out, wall clock), with the round-trip error."""
+    median of `reps`), the host-staged latency of the C++ drop-in header
+    (host vector in, host vector out, wall clock) and of the Python mirror
+    (numpy in, numpy out), with the round-trip error."""
     from fft_amd import _lib as L
     from fft_amd import synth
     n = 4096
@@ -260,8 +285,16 @@ def single_transform(torch, fft_amd, reps=200):
             f.ifft(yh, zh)
             hs.append((time.perf_counter() - t0) * 1e6)
         out[name] = {"device_round_trip_us": round(float(np.median(ts)), 2),
-                     "host_staged_round_trip_us": round(float(np.median(hs)), 1),
+                     "python_mirror_host_staged_round_trip_us": round(float(np.median(hs)), 1),
                      "round_trip_rel_l2": float(f"{rt:.3e}")}
+    fac = facade_round_trip(n, reps)
+    for name in ("f32", "f64"):
+        if name in fac:
+            out[name]["host_staged_round_trip_us"] = fac[name]["host_staged_round_trip_us"]
+            out[name]["facade_round_trip_rel_l2"] = fac[name]["round_trip_rel_l2"]
+    if "error" in fac:
+        out["facade_error"] = fac["error"]
+    out["host_staged_via"] = "C++ drop-in header (include/signalsmith-fft.h), std::vector in/out"
     return out
 
 
@@ -274,7 +307,8 @@ def precision_sweep(torch, fft_amd, reps=5):
     (32 N bytes FP64, 16 N FP32, 8 N 16-bit)."""
     from fft_amd import _lib as L
     sh = torch.cuda.current_stream().cuda_stream
-    out = {"note": "f16 / bf16 store 16-bit values and run float butterflies; the "
+    out = {"note": "f32x stores FP32 and computes in FP64 (twiddles, LDS, butterflies); "
+                   "f16 / bf16 store 16-bit values and run float butterflies; the "
                    "*_mfma_radix16 rows run the radix-16 stages on MFMA (split hi+lo operands)"}
     for n, batch, seed in ((1 << 20, 16, 20), (1 << 24, 1, 24)):
         g = torch.Generator(device="cuda").manual_seed(seed)
@@ -282,6 +316,7 @@ def precision_sweep(torch, fft_amd, reps=5):
         row = {}
         for name, prec, dt, eb, mfma in (
                 ("f64", L.F64, torch.float64, 8, -1), ("f32", L.F32, torch.float32, 4, -1),
+                ("f32x", L.F32X, torch.float32, 4, -1),
                 ("f16", L.F16, torch.float16, 2, -1), ("bf16", L.BF16, torch.bfloat16, 2, -1),
                 ("f16_mfma_radix16", L.F16, torch.float16, 2, 1),
                 ("bf16_mfma_radix16", L.BF16, torch.bfloat16, 2, 1)):
@@ -322,7 +357,7 @@ def serving(rates=(2000, 10000, 50000), duration=0.25):
     """Poisson arrivals: C2C fwd / inv / RealFFT fwd (1/3 each), fast sizes
     log-uniform in [256, 2^18], batch U{1..512}, per-request seeds; 64 GB
     HBM budget.  p50/p99 latency (arrival -> completion) and throughput, and
-    the plain path (one request at a time, synchronous) at the two lowest rates."""
+    the plain path (one request at a time, synchronous) at the same rates."""
     from fft_amd.serving import Server, latency_summary
     out = {}
     with Server(budget_bytes=64 << 30, max_inflight=64, streams=4) as srv:
@@ -332,7 +367,7 @@ def serving(rates=(2000, 10000, 50000), duration=0.25):
             out[f"rate_{rate}"] = {k: (round(v, 4) if isinstance(v, float) else v)
                                    for k, v in latency_summary(rec).items()}
     with Server(budget_bytes=64 << 30, plain=True) as srv:
-        for rate in rates[:2]:
+        for rate in rates:
             rec = srv.run_poisson(rate=rate, duration=duration, n_min=256, n_max=1 << 18,
                                   batch_max=512, seed=5)
             out[f"plain_rate_{rate}"] = {k: (round(v, 4) if isinstance(v, float) else v)
@@ -346,16 +381,30 @@ def cpu_baseline(torch, bufs, launches):
     transforms, repeated until about 16 thread-seconds of CPU work; plus rel L2
     of sample GPU outputs against the reference's FFT<double> / RealFFT<double>."""
     from oracle import pyoracle as po
-    threads = min(16, os.cpu_count() or 1)
-    out = {"threads": threads}
+    # every CPU this process may run on, capped by the host's CPU share per
+    # GPU where the host states one (OMP_NUM_THREADS)
+    avail = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", avail) or avail)
+    threads = max(1, min(avail, share))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    out = {"threads": threads, "nproc": os.cpu_count(), "affinity_cpus": avail,
+           "cpu_model": model}
     errors = {}
     sample_rows = np.linspace(0, C2C_BATCH - 1, 8).astype(int)
     if po.ref_available():
         kind = "reference"
         total_bytes, total_s = 0, 0.0
-        host = {n: (bufs[n][0].cpu().numpy(), bufs[n][1].cpu().numpy()) for n in C2C_SIZES}
-        xr, yr, zr_ = bufs["real"]
-        xrh, yrh = xr.cpu().numpy(), yr.cpu().numpy()
+        host = {n: (bufs[n][0].cpu().numpy(), bufs[n][3].cpu().numpy()) for n in C2C_SIZES}
+        xr, yr, zr_, yri = bufs["real"]
+        xrh, yrh = xr.cpu().numpy(), yri.cpu().numpy()
         # the step's transforms, repeated until about 16 thread-seconds of CPU work
         passes = 0
         while passes == 0 or total_s * threads < 16.0:
@@ -369,24 +418,25 @@ def cpu_baseline(torch, bufs, launches):
             total_bytes += 2 * 8 * REAL_N * REAL_FRAMES
             passes += 1
         for n in C2C_SIZES:
-            x, y, z = bufs[n]
-            xh, yh = host[n]
+            x, y, z, yi = bufs[n]
+            xh, yih = host[n]
             ref64 = po.ref_fft(xh[sample_rows].astype(np.complex128))
-            errors[f"c2c_{n}_fwd"] = po.rel_l2(yh[sample_rows], ref64)
+            errors[f"c2c_{n}_fwd"] = po.rel_l2(y[sample_rows].cpu().numpy(), ref64)
             zr = z[sample_rows].cpu().numpy()
             errors[f"c2c_{n}_inv"] = po.rel_l2(
-                zr, po.ref_fft(yh[sample_rows].astype(np.complex128), inverse=True))
+                zr, po.ref_fft(yih[sample_rows].astype(np.complex128), inverse=True))
         xh = xrh
         rows = np.linspace(0, REAL_FRAMES - 1, 8).astype(int)
         yh = yr[rows].cpu().numpy()
         errors[f"r2c_{REAL_N}_fwd"] = po.rel_l2(yh, po.ref_real_fft(xh[rows].astype(np.float64)))
         errors[f"c2r_{REAL_N}_inv"] = po.rel_l2(
-            zr_[rows].cpu().numpy(), po.ref_real_ifft(yh.astype(np.complex128)))
+            zr_[rows].cpu().numpy(), po.ref_real_ifft(yrh[rows].astype(np.complex128)))
         value = total_bytes / total_s / 1e9
         out.update({"value": round(value, 3), "unit": "GB/s", "cores": threads, "kind": kind,
                     "sample": f"the whole step {passes}x: every transform of the step, forward "
                               "and inverse (reference FFT<float>/RealFFT<float>, g++ -O3 "
-                              "-march=x86-64-v3), one instance per thread"})
+                              "-march=x86-64-v3), one instance per thread, on the host's CPU "
+                              "share for one GPU"})
     else:
         kind = "port"
         x = bufs[1920][0][:64].cpu().numpy()

@@ -55,109 +55,144 @@ void UnevenBatch::release() {
 }
 
 template <typename T>
-void UnevenBatch::build(size_t lds_cap) {
-    // One descriptor set per (plan, direction); the passes and radices are
-    // the plan's own, so every transform computes exactly what sfft_execute
-    // computes for it.  Only the slots per round change (tpw): the block is
-    // persistent_block<T>() threads wide.
-    std::vector<PassArgs<T>> table;
-    std::map<std::pair<const Plan *, int>, int> first_entry;
-    size_t lds_need = 0;
-    struct HostUnit {
-        WorkUnit u;
-        double cost;
-    };
-    std::vector<HostUnit> hu;
-    // single-pass transforms grouped per (plan, direction): element offsets
-    std::map<int, std::vector<std::pair<int64_t, int64_t>>> groups;
-    std::map<int, double> group_cost;
-    const size_t eb = sizeof(C2<T>);
-    for (size_t i = 0; i < items_.size(); ++i) {
-        const auto &it = items_[i];
-        const Plan &p = *plans_[i];
-        const int inv = it.direction;
-        auto key = std::make_pair(&p, inv);
-        if (!first_entry.count(key)) {
-            first_entry[key] = (int)table.size();
-            for (size_t q = 0; q < p.passes().size(); ++q) {
-                PassArgs<T> a = p.pass_args<T>(q, inv);
-                const size_t sl = p.slot_lds_bytes(q);
-                if (p.passes().size() == 1) {
-                    int tpw = persistent_block<T>() / a.pd.tpt;
-                    if (sl) tpw = std::min<int>(tpw, (int)(lds_cap / sl));
-                    a.pd.tpw = std::max(1, tpw);
-                } else {
-                    // column passes: as many adjacent columns per round as the
-                    // block and LDS allow; the last round of a transform masks
-                    // the columns past its end
-                    const int spt = p.passes()[q].slots_per_transform;
-                    int tpw = persistent_block<T>() / a.pd.tpt;
-                    if (sl) tpw = std::min<int>(tpw, (int)(lds_cap / sl));
-                    tpw = std::min(tpw, spt);
-                    int c = 1;
-                    while (c * 2 <= tpw) c *= 2;
-                    a.pd.tpw = std::max(a.pd.tpw, c);
-                }
-                lds_need = std::max(lds_need, sl * (size_t)a.pd.tpw);
-                table.push_back(a);
+int UnitTable<T>::entry(const Plan &p, int inv) {
+    const auto key = std::make_pair(&p, inv);
+    auto it = first_.find(key);
+    if (it != first_.end()) return it->second;
+    const int e0 = (int)descs_.size();
+    for (size_t q = 0; q < p.passes().size(); ++q) {
+        PassArgs<T> a = p.pass_args<T>(q, inv);
+        const size_t sl = p.slot_lds_bytes(q);
+        int tpw = persistent_block<T>() / a.pd.tpt;
+        if (sl) tpw = std::min<int>(tpw, (int)(lds_cap_ / sl));
+        if (p.passes().size() == 1) {
+            if (tpw < 2) {
+                // one slot per round (LDS-bound lengths): one butterfly per
+                // thread in every stage, up to the block (the bits do not
+                // depend on the thread layout)
+                int wide = 1;
+                for (int st = 0; st < a.pd.nstages; ++st)
+                    wide = std::max(wide, a.pd.n / a.pd.st[st].radix);
+                if (wide <= persistent_block<T>()) a.pd.tpt = std::max(a.pd.tpt, wide);
             }
+            a.pd.tpw = std::max(1, tpw);
+        } else {
+            // column passes: as many adjacent columns per round as the block
+            // and LDS allow; the last round of a transform masks the columns
+            // past its end
+            tpw = std::min(tpw, p.passes()[q].slots_per_transform);
+            int c = 1;
+            while (c * 2 <= tpw) c *= 2;
+            a.pd.tpw = std::max(a.pd.tpw, c);
         }
-        const int e0 = first_entry[key];
-        const int npass = (int)p.passes().size();
-        const size_t in_step = p.in_bytes(1, inv), out_step = p.out_bytes(1, inv);
-        if ((it.in_offset % eb) || (it.out_offset % eb) || (in_step % eb) || (out_step % eb))
-            throw Error(SFFT_ERR_INVALID_ARG, "batch item offsets must be element aligned");
-        for (size_t c = 0; c < it.count; ++c) {
-            const size_t io = it.in_offset + c * in_step, oo = it.out_offset + c * out_step;
-            if (npass == 1) {
-                groups[e0].push_back({(int64_t)(io / eb), (int64_t)(oo / eb)});
-                group_cost[e0] = (double)(in_step + out_step);
-            } else {
-                HostUnit h;
-                const int post = (it.kind != SFFT_C2C && inv == 0) ? kUnitRealPost : 0;
-                h.u = WorkUnit{e0, npass | post, 1, -1, (int64_t)io, (int64_t)oo};
-                h.cost = (double)(in_step + out_step) * npass;
-                hu.push_back(h);
-            }
+        lds_need_ = std::max(lds_need_, sl * (size_t)a.pd.tpw);
+        descs_.push_back(a);
+    }
+    first_[key] = e0;
+    return e0;
+}
+
+template <typename T>
+void UnitBuilder<T>::add(UnitTable<T> &table, const Plan &p, int kind, int inv, size_t count,
+                         size_t in_off, size_t out_off) {
+    const int e0 = table.entry(p, inv);
+    const int npass = (int)p.passes().size();
+    const size_t eb = sizeof(C2<T>);
+    const size_t in_step = p.in_bytes(1, inv), out_step = p.out_bytes(1, inv);
+    if ((in_off % eb) || (out_off % eb) || (in_step % eb) || (out_step % eb))
+        throw Error(SFFT_ERR_INVALID_ARG, "batch item offsets must be element aligned");
+    for (size_t c = 0; c < count; ++c) {
+        const size_t io = in_off + c * in_step, oo = out_off + c * out_step;
+        if (npass == 1) {
+            groups_[e0].push_back({(int64_t)(io / eb), (int64_t)(oo / eb)});
+            group_cost_[e0] = (double)(in_step + out_step);
+            group_tpw_[e0] = table.descs()[(size_t)e0].pd.tpw;
+        } else {
+            HostUnit h;
+            const int post = (kind != SFFT_C2C && inv == 0) ? kUnitRealPost : 0;
+            h.u = WorkUnit{e0, npass | post, 1, -1, (int64_t)io, (int64_t)oo};
+            h.cost = (double)(in_step + out_step) * npass;
+            hu_.push_back(h);
         }
     }
-    std::vector<int64_t> gin, gout;
-    for (auto &g : groups) {
+}
+
+template <typename T>
+void UnitBuilder<T>::finish(std::vector<WorkUnit> &units, std::vector<int64_t> &gin,
+                            std::vector<int64_t> &gout) {
+    gin.clear();
+    gout.clear();
+    for (auto &g : groups_) {
         const int e0 = g.first;
-        const size_t per_unit = (size_t)table[e0].pd.tpw;
+        const size_t per_unit = (size_t)group_tpw_[e0];
         const auto &lst = g.second;
         for (size_t c = 0; c < lst.size(); c += per_unit) {
             const size_t cnt = std::min(per_unit, lst.size() - c);
             HostUnit h;
             h.u = WorkUnit{e0, 1, (int32_t)cnt, (int32_t)gin.size(), 0, 0};
-            h.cost = (double)cnt * group_cost[e0];
+            h.cost = (double)cnt * group_cost_[e0];
             for (size_t k = 0; k < cnt; ++k) {
                 gin.push_back(lst[c + k].first);
                 gout.push_back(lst[c + k].second);
             }
-            hu.push_back(h);
+            hu_.push_back(h);
         }
     }
     // Longest first: big units start early, small ones fill the tail.
-    std::stable_sort(hu.begin(), hu.end(),
+    std::stable_sort(hu_.begin(), hu_.end(),
                      [](const HostUnit &a, const HostUnit &b) { return a.cost > b.cost; });
+    units.clear();
+    units.reserve(hu_.size());
+    for (auto &h : hu_) units.push_back(h.u);
+    hu_.clear();
+    groups_.clear();
+    group_cost_.clear();
+    group_tpw_.clear();
+}
+
+template <typename T>
+unsigned persistent_grid(size_t lds_bytes, size_t nunits) {
+    const int per_cu = persistent_blocks_per_cu<T>(lds_bytes);
+    unsigned grid = (unsigned)(device_info().cu_count * per_cu);
+    if (nunits && grid > nunits) grid = (unsigned)nunits;
+    return grid ? grid : 1;
+}
+
+template class UnitTable<float>;
+template class UnitTable<double>;
+template class UnitBuilder<float>;
+template class UnitBuilder<double>;
+template unsigned persistent_grid<float>(size_t, size_t);
+template unsigned persistent_grid<double>(size_t, size_t);
+
+template <typename T>
+void UnevenBatch::build(size_t lds_cap) {
+    // One descriptor set per (plan, direction); the passes and radices are
+    // the plan's own, so every transform computes exactly what sfft_execute
+    // computes for it.  Only the slots per round change (tpw): the block is
+    // persistent_block<T>() threads wide.
+    UnitTable<T> table(lds_cap);
+    UnitBuilder<T> builder;
+    for (size_t i = 0; i < items_.size(); ++i) {
+        const auto &it = items_[i];
+        builder.add(table, *plans_[i], it.kind, it.direction, it.count, it.in_offset,
+                    it.out_offset);
+    }
     std::vector<WorkUnit> units;
-    units.reserve(hu.size());
-    for (auto &h : hu) units.push_back(h.u);
+    std::vector<int64_t> gin, gout;
+    builder.finish(units, gin, gout);
     nunits_ = units.size();
-    lds_bytes_ = lds_need;
-    const int per_cu = persistent_blocks_per_cu<T>(lds_bytes_);
-    grid_ = (unsigned)(device_info().cu_count * per_cu);
-    if (nunits_ && grid_ > nunits_) grid_ = (unsigned)nunits_;
-    if (grid_ == 0) grid_ = 1;
+    lds_bytes_ = table.lds_need();
+    grid_ = persistent_grid<T>(lds_bytes_, nunits_);
 
     auto upload = [](void **dst, const void *src, size_t bytes, const char *what) {
         if (hipMalloc(dst, bytes ? bytes : 16) != hipSuccess)
             throw Error(SFFT_ERR_OUT_OF_MEMORY, std::string(what) + " allocation failed");
         if (bytes) check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
     };
+    const auto &table_h = table.descs();
     upload((void **)&d_units_, units.data(), units.size() * sizeof(WorkUnit), "units");
-    upload(&d_table_, table.data(), table.size() * sizeof(PassArgs<T>), "descriptor table");
+    upload(&d_table_, table_h.data(), table_h.size() * sizeof(PassArgs<T>), "descriptor table");
     upload((void **)&d_gather_in_, gin.data(), gin.size() * sizeof(int64_t), "gather list");
     upload((void **)&d_gather_out_, gout.data(), gout.size() * sizeof(int64_t), "gather list");
     upload((void **)&d_counters_, nullptr, 0, "counters");

@@ -4,14 +4,61 @@
 #include <hip/hip_runtime.h>
 #include <sfft.h>
 
+#include <map>
 #include <memory>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "engine.h"
 #include "persistent.h"
 
 namespace sfft {
+
+// Work list of one persistent launch.  Transforms are added one item at a
+// time; single-pass ones are gathered per (plan, direction) into units of
+// one workgroup round, two-pass ones are a unit each; finish() orders the
+// units longest first.  The pass descriptors come from a UnitTable, so a
+// transform computes exactly what sfft_execute computes for it (same passes,
+// radices and tables; only the slots per round differ).
+template <typename T>
+class UnitTable {
+  public:
+    explicit UnitTable(size_t lds_cap) : lds_cap_(lds_cap) {}
+    // First descriptor index of (plan, direction), appending its passes.
+    int entry(const Plan &p, int inverse);
+    const std::vector<PassArgs<T>> &descs() const { return descs_; }
+    size_t lds_need() const { return lds_need_; }
+
+  private:
+    size_t lds_cap_;
+    size_t lds_need_ = 0;
+    std::vector<PassArgs<T>> descs_;
+    std::map<std::pair<const Plan *, int>, int> first_;
+};
+
+template <typename T>
+class UnitBuilder {
+  public:
+    // `count` transforms of plan p in direction `inverse`, the first at byte
+    // offsets in_off / out_off of the arenas, the rest back to back.
+    void add(UnitTable<T> &table, const Plan &p, int kind, int inverse, size_t count,
+             size_t in_off, size_t out_off);
+    // Units longest first plus the gather lists (element offsets).
+    void finish(std::vector<WorkUnit> &units, std::vector<int64_t> &gin,
+                std::vector<int64_t> &gout);
+    bool empty() const { return hu_.empty() && groups_.empty(); }
+
+  private:
+    struct HostUnit {
+        WorkUnit u;
+        double cost;
+    };
+    std::vector<HostUnit> hu_;
+    std::map<int, std::vector<std::pair<int64_t, int64_t>>> groups_;
+    std::map<int, double> group_cost_;
+    std::map<int, int> group_tpw_;
+};
 
 class UnevenBatch {
   public:
@@ -59,5 +106,9 @@ class UnevenBatch {
     const void *graph_in_ = nullptr;
     void *graph_out_ = nullptr;
 };
+
+// Workgroups of a persistent launch whose descriptors need lds_bytes of LDS.
+template <typename T>
+unsigned persistent_grid(size_t lds_bytes, size_t nunits);
 
 }  // namespace sfft

@@ -197,6 +197,8 @@ static bool layout_single(int n, int prec, bool uses_lds_always, const PlanHints
             for (int r : h->radices) prod *= r;
             if (prod != n) return false;
             radices = h->radices;
+            // radix-32 stages run in the 32-element instances only
+            if (e < kEptLarge && *std::max_element(radices.begin(), radices.end()) > 25) continue;
             if (threads_for(n, radices, e) > max_block(prec)) continue;
         } else if (!choose_radices(n, e, radices, arith_f64(prec) ? 16 : 25, max_block(prec))) {
             continue;

@@ -8,18 +8,16 @@ namespace sfft {
 namespace {
 // 1920: 192 threads per transform give every stage one butterfly per thread
 // (120, 160 and 192 butterflies); 160 threads leave the radix-10 stage with two
-// butterflies on a fifth of the threads, measured 10-20 % slower.
-const FixedC2C<float, 1920, 192, 2, Sched<0, 4, 0>, 0, 16, 12, 10> c1920a;
+// butterflies on a fifth of the threads, measured 20 % slower.  One transform
+// per block measured fastest on inputs the last-level cache does not hold
+// (4.69 TB/s; two per block 4.45, four 4.17; 128 threads with two stages of
+// two butterflies 4.57, 64 threads 4.43).
+const FixedC2C<float, 1920, 192, 1, Sched<0, 4, 0>, 0, 16, 12, 10> c1920c;
 const FixedC2C<float, 1920, 192, 2, Sched<0, 0, 0>, 1, 16, 12, 10> c1920b;
-const FixedC2C<float, 1920, 192, 1, Sched<0, 4, 0>, 2, 16, 12, 10> c1920c;
+const FixedC2C<float, 1920, 192, 2, Sched<0, 4, 0>, 2, 16, 12, 10> c1920a;
 const FixedC2C<float, 1920, 160, 2, Sched<0, 4, 0>, 3, 16, 12, 10> c1920d;
-const FixedC2C<float, 1920, 160, 2, Sched<0, 0, 0>, 4, 16, 12, 10> c1920e;
-const FixedC2C<float, 1920, 128, 2, Sched<0, 0, 0>, 5, 16, 12, 10> c1920f;
-const FixedC2C<float, 1920, 128, 4, Sched<0, 0, 0>, 6, 16, 12, 10> c1920g;
-const FixedC2C<float, 1920, 64, 4, Sched<0, 0, 0>, 7, 16, 12, 10> c1920h;
-const FixedC2C<float, 1920, 96, 4, Sched<0, 0, 0>, 8, 16, 12, 10> c1920i;
-const FixedC2C<float, 1920, 192, 4, Sched<0, 4, 0>, 9, 16, 12, 10> c1920j;
-const FixedC2C<float, 1920, 192, 1, Sched<0, 8, 0>, 10, 16, 12, 10> c1920k;
+const FixedC2C<float, 1920, 128, 2, Sched<0, 0, 0>, 4, 16, 12, 10> c1920f;
+const FixedC2C<float, 1920, 64, 4, Sched<0, 0, 0>, 5, 16, 12, 10> c1920h;
 // 3000: four stages (250, 300, 600, 600 butterflies on 300 threads: 10-12
 // elements per thread in every stage) measured 2 % faster than three with a
 // radix-25 stage that only 120 of 250 threads work in.  Its LDS exchanges have
@@ -29,9 +27,8 @@ const FixedC2C<float, 3000, 300, 1, Sched<0, 4, 0, 0, kLdsNoPad>, 0, 12, 10, 5, 
 const FixedC2C<float, 3000, 300, 1, Sched<0, 4, 0>, 1, 12, 10, 5, 5> c3000b;
 const FixedC2C<float, 3000, 250, 1, Sched<0, 0, 0>, 2, 12, 10, 25> c3000c;
 const FixedC2C<float, 3000, 250, 1, Sched<-1, 0, 0>, 3, 12, 10, 25> c3000d;
-const FixedC2C<float, 3000, 150, 2, Sched<0, 0, 0, 0, kLdsNoPad>, 4, 12, 10, 5, 5> c3000e;
-const FixedC2C<float, 3000, 300, 2, Sched<0, 4, 0, 0, kLdsNoPad>, 5, 12, 10, 5, 5> c3000f;
-const FixedC2C<float, 3000, 100, 4, Sched<0, 0, 0, 0, kLdsNoPad>, 6, 12, 10, 5, 5> c3000g;
-const FixedC2C<float, 3000, 150, 4, Sched<0, 0, 0, 0, kLdsNoPad>, 7, 12, 10, 5, 5> c3000h;
+// fewer threads per transform, or more transforms per block, measured slower
+// (150 x 2: 3.58 TB/s, 300 x 2: 3.77, 100 x 4: 3.27, against 4.42)
+const FixedC2C<float, 3000, 300, 2, Sched<0, 4, 0, 0, kLdsNoPad>, 4, 12, 10, 5, 5> c3000f;
 }  // namespace
 }  // namespace sfft

@@ -462,6 +462,9 @@ struct Block {
         case 15: stage<15>(s, src_global, dst_global, col_map); break;
         case 16: stage<16>(s, src_global, dst_global, col_map); break;
         case 25: stage<25>(s, src_global, dst_global, col_map); break;
+        case 32:  // compiled 16384 plans; only in the 32-element instances
+            if constexpr (EPT >= 32) stage<32>(s, src_global, dst_global, col_map);
+            break;
         default: break;
         }
     }
