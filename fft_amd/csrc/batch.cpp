@@ -14,14 +14,21 @@ UnevenBatch::UnevenBatch(const sfft_batch_item *items, size_t count, int prec)
         if (it.direction != SFFT_FORWARD && it.direction != SFFT_INVERSE)
             throw Error(SFFT_ERR_INVALID_ARG, "batch item direction");
         plans_.push_back(get_plan(it.n, it.kind, prec));
+        // chirp-z lengths (prime factors above 13) run as their plan's own
+        // launch sequence after the persistent launch, in the same graph,
+        // through one workspace sized for the largest of them
         if (plans_.back()->bluestein())
-            throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
-                        "size " + std::to_string(it.n) +
-                            ": lengths with prime factors > 13 run through sfft_execute only");
+            ws_bytes_ = std::max(ws_bytes_, plans_.back()->workspace_bytes(it.count, false));
     }
+
     const size_t lds_cap = device_info().lds_bytes - kLdsReserve;
     check_hip(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "batch event");
     try {
+        if (ws_bytes_ && hipMalloc(&d_ws_, ws_bytes_) != hipSuccess) {
+            (void)hipGetLastError();
+            d_ws_ = nullptr;
+            throw Error(SFFT_ERR_OUT_OF_MEMORY, "chirp-z workspace allocation failed");
+        }
         if (prec == kF64)
             build<double>(lds_cap);
         else
@@ -46,6 +53,8 @@ void UnevenBatch::release() {
     if (d_counters_) (void)hipFree(d_counters_);
     if (d_gather_in_) (void)hipFree(d_gather_in_);
     if (d_gather_out_) (void)hipFree(d_gather_out_);
+    if (d_ws_) (void)hipFree(d_ws_);
+    d_ws_ = nullptr;
     graph_exec_ = nullptr;
     graph_ = nullptr;
     d_units_ = nullptr;
@@ -175,6 +184,7 @@ void UnevenBatch::build(size_t lds_cap) {
     UnitBuilder<T> builder;
     for (size_t i = 0; i < items_.size(); ++i) {
         const auto &it = items_[i];
+        if (plans_[i]->bluestein()) continue;
         builder.add(table, *plans_[i], it.kind, it.direction, it.count, it.in_offset,
                     it.out_offset);
     }
@@ -214,6 +224,14 @@ void UnevenBatch::launch(const void *in_arena, void *out_arena, hipStream_t stre
                                            d_gather_out_, d_counters_,
                                            in_arena, out_arena, grid_, lds_bytes_, stream),
                   what.c_str());
+    for (size_t k = 0; k < items_.size(); ++k) {
+        const auto &it = items_[k];
+        const Plan &p = *plans_[k];
+        if (!p.bluestein()) continue;
+        p.execute((const char *)in_arena + it.in_offset, p.default_in_dist(it.direction),
+                  (char *)out_arena + it.out_offset, p.default_out_dist(it.direction), it.count,
+                  it.direction, stream, d_ws_, ws_bytes_);
+    }
 }
 
 void UnevenBatch::execute_persistent(const void *in_arena, void *out_arena, hipStream_t stream) {

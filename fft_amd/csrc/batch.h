@@ -67,11 +67,12 @@ class UnevenBatch {
     UnevenBatch(const UnevenBatch &) = delete;
     UnevenBatch &operator=(const UnevenBatch &) = delete;
 
-    // One persistent launch over every transform of every item, replayed
-    // from a captured hipGraph (re-captured when the arenas change).  The
-    // launch's work queue lives in the batch, so launches of one batch are
-    // serialised on the device: each waits for the previous one (on any
-    // stream, from any thread) before it starts.
+    // One persistent launch over every transform of every item (items of
+    // chirp-z lengths follow it as their plan's own launches, in the same
+    // graph), replayed from a captured hipGraph (re-captured when the arenas
+    // change).  The launch's work queue lives in the batch, so launches of
+    // one batch are serialised on the device: each waits for the previous one
+    // (on any stream, from any thread) before it starts.
     void execute_persistent(const void *in_arena, void *out_arena, hipStream_t stream);
     // The plain per-size loop: one batched launch sequence per item.
     void execute_per_size(const void *in_arena, void *out_arena, hipStream_t stream);
@@ -96,6 +97,9 @@ class UnevenBatch {
     size_t nunits_ = 0;
     unsigned grid_ = 0;
     size_t lds_bytes_ = 0;
+    // workspace of the chirp-z items (run after the persistent launch)
+    void *d_ws_ = nullptr;
+    size_t ws_bytes_ = 0;
     // captured launch; mu_ guards the graph and the ordering event, which
     // marks the end of the last launch (launched_: recorded at least once)
     std::mutex mu_;

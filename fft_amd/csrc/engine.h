@@ -137,6 +137,9 @@ class Plan {
     void run(const void *in, size_t in_dist, void *out, size_t out_dist, size_t batch, int inverse,
              hipStream_t stream, void *ws) const;
     template <typename T>
+    void run_passes(const void *in, int64_t din, void *out, int64_t dout, size_t batch,
+                    int inverse, hipStream_t stream, void *ws) const;
+    template <typename T>
     void run_fixed(const void *in, void *out, size_t batch, int inverse, hipStream_t stream) const;
     template <typename T>
     void run_fixed_columns(size_t p, const void *src, int64_t dsrc, void *dst, int64_t ddst,
@@ -183,6 +186,8 @@ std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec);
 bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix = 25,
                     int max_threads = 1024);
 bool supported_length(size_t n);
+// Intermediate bytes per chunk of a two-pass batch (0: the whole batch at once).
+size_t two_pass_chunk_bytes();
 // Any length a plan exists for in F32 / F64: direct, or chirp-z (up to 2^27).
 bool plannable_length(size_t n);
 

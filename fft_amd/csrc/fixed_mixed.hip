@@ -18,6 +18,8 @@ const FixedC2C<float, 1920, 192, 2, Sched<0, 4, 0>, 2, 16, 12, 10> c1920a;
 const FixedC2C<float, 1920, 160, 2, Sched<0, 4, 0>, 3, 16, 12, 10> c1920d;
 const FixedC2C<float, 1920, 128, 2, Sched<0, 0, 0>, 4, 16, 12, 10> c1920f;
 const FixedC2C<float, 1920, 64, 4, Sched<0, 0, 0>, 5, 16, 12, 10> c1920h;
+const FixedC2C<float, 1920, 192, 1, Sched<0, 4, 0, 0, 5>, 6, 16, 12, 10> c1920p5;
+const FixedC2C<float, 1920, 192, 1, Sched<0, 4, 0, 0, kLdsNoPad>, 7, 16, 12, 10> c1920np;
 // 3000: four stages (250, 300, 600, 600 butterflies on 300 threads: 10-12
 // elements per thread in every stage) measured 2 % faster than three with a
 // radix-25 stage that only 120 of 250 threads work in.  Its LDS exchanges have
@@ -30,5 +32,7 @@ const FixedC2C<float, 3000, 250, 1, Sched<-1, 0, 0>, 3, 12, 10, 25> c3000d;
 // fewer threads per transform, or more transforms per block, measured slower
 // (150 x 2: 3.58 TB/s, 300 x 2: 3.77, 100 x 4: 3.27, against 4.42)
 const FixedC2C<float, 3000, 300, 2, Sched<0, 4, 0, 0, kLdsNoPad>, 4, 12, 10, 5, 5> c3000f;
+const FixedC2C<float, 3000, 300, 1, Sched<0, 4, 0, 0, 3>, 5, 12, 10, 5, 5> c3000p3;
+const FixedC2C<float, 3000, 300, 1, Sched<0, 4, 0, 0, 5>, 6, 12, 10, 5, 5> c3000p5;
 }  // namespace
 }  // namespace sfft

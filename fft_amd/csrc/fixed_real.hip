@@ -13,5 +13,7 @@ const FixedReal<float, 1024, 64, 2, Sched<0, 0, 0>, 0, 16, 16, 4> r2048e;
 const FixedReal<float, 1024, 64, 4, Sched<-1, 0, 0>, 1, 16, 16, 4> r2048b;
 const FixedReal<float, 1024, 64, 4, Sched<0, 4, 0>, 2, 16, 16, 4> r2048c;
 const FixedReal<float, 1024, 64, 4, Sched<0, 0, 0>, 3, 16, 16, 4> r2048a;
+const FixedReal<float, 1024, 64, 2, Sched<0, 0, 0, 0, 5>, 4, 16, 16, 4> r2048p5;
+const FixedReal<float, 1024, 64, 2, Sched<0, 0, 0, 0, 3>, 5, 16, 16, 4> r2048p3;
 }  // namespace
 }  // namespace sfft

@@ -166,10 +166,17 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+__host__ __device__ __forceinline__ uint64_t fill_key(uint64_t seed) {
+    uint64_t z = seed;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 template <typename T>
-__global__ void k_fill(T *dst, uint64_t count, int dist, uint64_t key) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += stride) {
+__device__ __forceinline__ void fill_range(T *dst, uint64_t count, int dist, uint64_t key,
+                                           uint64_t first, uint64_t stride) {
+    for (uint64_t i = first; i < count; i += stride) {
         if (dist == 0) {
             const uint64_t w = mix64(key + (i + 1) * 0x9E3779B97F4A7C15ull);
             const double u = (double)(w >> 40) * (1.0 / 16777216.0);
@@ -189,14 +196,34 @@ __global__ void k_fill(T *dst, uint64_t count, int dist, uint64_t key) {
     }
 }
 
+template <typename T>
+__global__ void k_fill(T *dst, uint64_t count, int dist, uint64_t key) {
+    fill_range(dst, count, dist, key, blockIdx.x * (uint64_t)blockDim.x + threadIdx.x,
+               (uint64_t)gridDim.x * blockDim.x);
+}
+
+// One buffer per blockIdx.y.
+__global__ void k_fill_many(const FillEntry *entries) {
+    const FillEntry e = entries[blockIdx.y];
+    const uint64_t first = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t key = fill_key(e.seed);
+    if (e.f64)
+        fill_range(static_cast<double *>(e.dst), e.count, e.dist, key, first, stride);
+    else
+        fill_range(static_cast<float *>(e.dst), e.count, e.dist, key, first, stride);
+}
+
+hipError_t fill_synthetic_many(const FillEntry *entries, int n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_many, dim3(256, (unsigned)n), dim3(256), 0, stream, entries);
+    return hipGetLastError();
+}
+
 hipError_t fill_synthetic(void *dst, size_t count, int f64, int dist, uint64_t seed,
                           hipStream_t stream) {
     if (count == 0) return hipSuccess;
-    const uint64_t key = [](uint64_t z) {
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-        return z ^ (z >> 31);
-    }(seed);
+    const uint64_t key = fill_key(seed);
     size_t blocks = (count + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (f64)

@@ -1,32 +1,40 @@
 // serve.cpp -- serving loop: continuous batching of transform requests.
 //
-// A worker thread owns the device side.  Each tick it
-//   1. retires requests whose completion event has fired (frees their
-//      buffers unless retained, records the completion time),
-//   2. admits queued requests in arrival order while an in-flight slot is
-//      free and their input + output buffers fit in the HBM pool (the head
-//      of the queue blocks the rest: no reordering, no starvation),
-//   3. enqueues each admitted request on the next device stream: synthesise
-//      the input from its seed, run the plan, record an event.
-// Admitted requests therefore overlap on the device, new arrivals are
-// admitted as soon as slots and memory free up, and plans / twiddles stay
-// resident in the plan cache.  The plain mode serves one request at a time
-// synchronously (the comparison path).
+// A worker thread admits queued requests in arrival order while an in-flight
+// slot is free and their buffers (input, output and any workspace) fit in the
+// HBM pool; the head of the queue blocks the rest (no reordering, no
+// starvation).  Everything admitted in one pass is launched together:
+//   * small requests (each at most kMergeBytes of input + output) are merged
+//     into ONE persistent launch (the uneven-batch kernel over a per-tick
+//     work list, with one fill launch synthesising all their inputs), so many
+//     requests of different sizes, kinds and directions cost two launches;
+//   * larger requests run their plan's own kernels (compiled where one
+//     exists), each with its own fill, round-robin over the device streams.
+// Each stream has a waiter thread that blocks on the completion events of
+// its requests in launch order and retires them (frees their buffers unless
+// retained, records the completion time), which wakes the worker to admit
+// more.  Plans, twiddles and descriptor tables stay device-resident.  The
+// plain mode serves one request at a time synchronously (the comparison
+// path).
 #include <sfft.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <condition_variable>
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
+#include "batch.h"
 #include "engine.h"
+#include "kernels.h"
 
 namespace sfft {
 
@@ -89,16 +97,26 @@ class Pool {
 struct Req {
     sfft_request r;
     double t_arrive = 0, t_admit = 0, t_done = 0;
-    size_t in_bytes = 0, out_bytes = 0, in_off = SIZE_MAX, out_off = SIZE_MAX;
+    size_t in_bytes = 0, out_bytes = 0, ws_bytes = 0;
+    size_t in_off = SIZE_MAX, out_off = SIZE_MAX, ws_off = SIZE_MAX;
     std::shared_ptr<Plan> plan;
     hipEvent_t ev = nullptr;
     int status = SFFT_OK;
 };
 
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
 }  // namespace
 
 class Server {
   public:
+    // A request of at most this many input + output bytes joins the tick's
+    // merged persistent launch.
+    static constexpr size_t kMergeBytes = 16ull << 20;
+    // Capacity of one stream's merged work list.
+    static constexpr size_t kMaxUnits = 1 << 16, kMaxGather = 1 << 17, kMaxFills = 1024;
+    static constexpr size_t kMaxDescs = 8192;
+
     explicit Server(const sfft_server_config &c) : cfg_(c) {
         if (cfg_.hbm_budget_bytes == 0) cfg_.hbm_budget_bytes = 64ull << 30;
         if (cfg_.max_inflight <= 0) cfg_.max_inflight = 64;
@@ -110,12 +128,15 @@ class Server {
         if (cfg_.precision != kF32 && cfg_.precision != kF64)
             throw Error(SFFT_ERR_INVALID_ARG, "precision");
         pool_ = std::make_unique<Pool>(cfg_.hbm_budget_bytes);
-        for (int i = 0; i < cfg_.streams; ++i) {
-            hipStream_t s;
-            check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream create");
-            streams_.push_back(s);
+        try {
+            setup_device();
+        } catch (...) {
+            teardown_device();
+            throw;
         }
         t0_ = std::chrono::steady_clock::now();
+        if (!cfg_.plain)
+            for (auto &st : streams_) st->waiter = std::thread([this, p = st.get()] { wait_loop(p); });
         worker_ = std::thread([this] { loop(); });
     }
 
@@ -126,10 +147,16 @@ class Server {
         }
         cv_.notify_all();
         worker_.join();
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_waiters_ = true;
+        }
+        for (auto &st : streams_) {
+            st->cv.notify_all();
+            if (st->waiter.joinable()) st->waiter.join();
+        }
         for (auto &kv : retained_) pool_->release(kv.second.first, kv.second.second);
-        for (Req *r : inflight_) finish_wait(r);
-        for (auto s : streams_) (void)hipStreamDestroy(s);
-        for (auto e : event_pool_) (void)hipEventDestroy(e);
+        teardown_device();
     }
 
     double now() const {
@@ -144,8 +171,11 @@ class Server {
             r->plan = get_plan(rq.n, rq.kind, cfg_.precision);
             r->in_bytes = r->plan->in_bytes(rq.batch, rq.direction);
             r->out_bytes = r->plan->out_bytes(rq.batch, rq.direction);
-            const size_t need = ((r->in_bytes + 255) & ~size_t(255)) + ((r->out_bytes + 255) & ~size_t(255));
-            if (need > pool_->size()) throw Error(SFFT_ERR_BUDGET, "request larger than the HBM budget");
+            r->ws_bytes = r->plan->workspace_bytes(rq.batch, false);
+            const size_t need =
+                align256(r->in_bytes) + align256(r->out_bytes) + align256(r->ws_bytes);
+            if (need > pool_->size())
+                throw Error(SFFT_ERR_BUDGET, "request larger than the HBM budget");
         } catch (const Error &e) {
             r->status = e.code;
         }
@@ -198,15 +228,86 @@ class Server {
     }
 
   private:
+    // One device stream, its launched-but-not-retired requests (in launch
+    // order) and the buffers of its merged launches.
+    struct StreamCtx {
+        hipStream_t s = nullptr;
+        std::deque<Req *> fifo;  // guarded by the server's mu_
+        std::condition_variable cv;
+        std::thread waiter;
+        unsigned *d_counters = nullptr;  // persistent-launch queue counters
+        char *h_block = nullptr;         // pinned staging of the work list
+        char *d_block = nullptr;         // its device copy
+        hipEvent_t copy_done = nullptr;  // the last staging copy has been read
+        bool copied = false;
+    };
+    static constexpr size_t kBlockBytes =
+        kMaxUnits * sizeof(WorkUnit) + 2 * kMaxGather * sizeof(int64_t) +
+        kMaxFills * sizeof(FillEntry);
+
+    void setup_device() {
+        for (int i = 0; i < cfg_.streams; ++i) {
+            streams_.push_back(std::make_unique<StreamCtx>());
+            StreamCtx &st = *streams_.back();
+            check_hip(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking), "stream create");
+            if (cfg_.plain) continue;
+            if (hipMalloc((void **)&st.d_counters, 2 * sizeof(unsigned)) != hipSuccess ||
+                hipMalloc((void **)&st.d_block, kBlockBytes) != hipSuccess ||
+                hipHostMalloc((void **)&st.h_block, kBlockBytes, hipHostMallocDefault) !=
+                    hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(SFFT_ERR_OUT_OF_MEMORY, "serving work-list buffers");
+            }
+            check_hip(hipMemset(st.d_counters, 0, 2 * sizeof(unsigned)), "counter init");
+            check_hip(hipEventCreateWithFlags(&st.copy_done, hipEventDisableTiming), "event");
+        }
+        if (cfg_.plain) return;
+        const size_t eb = cfg_.precision == kF64 ? sizeof(PassArgs<double>) : sizeof(PassArgs<float>);
+        if (hipMalloc(&d_table_, kMaxDescs * eb) != hipSuccess ||
+            hipHostMalloc(&h_table_, kMaxDescs * eb, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(SFFT_ERR_OUT_OF_MEMORY, "serving descriptor table");
+        }
+        const size_t lds_cap = device_info().lds_bytes - kLdsReserve;
+        if (cfg_.precision == kF64)
+            table64_ = std::make_unique<UnitTable<double>>(lds_cap);
+        else
+            table32_ = std::make_unique<UnitTable<float>>(lds_cap);
+    }
+
+    void teardown_device() {
+        for (auto &st : streams_) {
+            if (st->s) (void)hipStreamSynchronize(st->s);
+            if (st->d_counters) (void)hipFree(st->d_counters);
+            if (st->d_block) (void)hipFree(st->d_block);
+            if (st->h_block) (void)hipHostFree(st->h_block);
+            if (st->copy_done) (void)hipEventDestroy(st->copy_done);
+            if (st->s) (void)hipStreamDestroy(st->s);
+        }
+        streams_.clear();
+        if (d_table_) (void)hipFree(d_table_);
+        if (h_table_) (void)hipHostFree(h_table_);
+        d_table_ = h_table_ = nullptr;
+        for (auto e : event_pool_) (void)hipEventDestroy(e);
+        event_pool_.clear();
+    }
+
     hipEvent_t get_event() {
-        if (!event_pool_.empty()) {
-            hipEvent_t e = event_pool_.back();
-            event_pool_.pop_back();
-            return e;
+        {
+            std::lock_guard<std::mutex> lock(ev_mu_);
+            if (!event_pool_.empty()) {
+                hipEvent_t e = event_pool_.back();
+                event_pool_.pop_back();
+                return e;
+            }
         }
         hipEvent_t e;
         check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event create");
         return e;
+    }
+    void put_event(hipEvent_t e) {
+        std::lock_guard<std::mutex> lock(ev_mu_);
+        event_pool_.push_back(e);
     }
 
     void complete(Req *r) {  // mu_ held
@@ -218,129 +319,270 @@ class Server {
         c.t_done = r->t_done;
         c.bytes = r->status == SFFT_OK ? r->in_bytes + r->out_bytes : 0;
         done_.push_back(c);
-        if (r->ev) event_pool_.push_back(r->ev);
+        if (r->ev) put_event(r->ev);
         delete r;
         if (--outstanding_ == 0) idle_cv_.notify_all();
     }
 
-    void finish_wait(Req *r) {
-        if (r->ev) (void)hipEventSynchronize(r->ev);
+    // Frees a finished request's buffers (keeping a retained output) and
+    // records its completion.  mu_ held.
+    void retire(Req *r, hipError_t q) {
+        r->t_done = now();
+        if (q != hipSuccess) {
+            (void)hipGetLastError();
+            r->status = SFFT_ERR_DEVICE;
+        }
+        pool_->release(r->in_off, r->in_bytes);
+        if (r->ws_off != SIZE_MAX) pool_->release(r->ws_off, r->ws_bytes);
+        if (r->r.retain && r->status == SFFT_OK)
+            retained_[r->r.id] = {r->out_off, r->out_bytes};
+        else
+            pool_->release(r->out_off, r->out_bytes);
+        --inflight_;
+        complete(r);
     }
 
-    // enqueue the device work of an admitted request (worker thread, mu_ not
-    // held: uses only the streams, the event pool and the request)
-    void launch(Req *r) {
-        hipStream_t s = streams_[next_stream_++ % streams_.size()];
+    // Waiter of one stream: blocks on its requests' events in launch order.
+    void wait_loop(StreamCtx *st) {
+        std::unique_lock<std::mutex> lock(mu_);
+        while (true) {
+            st->cv.wait(lock, [&] { return !st->fifo.empty() || stop_waiters_; });
+            if (st->fifo.empty()) break;
+            Req *r = st->fifo.front();
+            lock.unlock();
+            const hipError_t q = hipEventSynchronize(r->ev);
+            lock.lock();
+            st->fifo.pop_front();
+            retire(r, q);
+            cv_.notify_all();  // memory and a slot are free: admit more
+        }
+    }
+
+    // Device work of one request on its own: fill, then the plan's kernels.
+    void launch_single(Req *r, hipStream_t s) {
         char *in = pool_->base() + r->in_off;
         char *out = pool_->base() + r->out_off;
+        void *ws = r->ws_off != SIZE_MAX ? pool_->base() + r->ws_off : nullptr;
         const size_t reals = r->in_bytes / (cfg_.precision == kF64 ? 8 : 4);
-        try {
-            check_hip(fill_synthetic(in, reals, cfg_.precision == kF64, 0, r->r.seed, s),
-                      "fill");
-            r->plan->execute(in, r->plan->default_in_dist(r->r.direction), out,
-                             r->plan->default_out_dist(r->r.direction), r->r.batch,
-                             r->r.direction, s);
-            r->ev = get_event();
-            check_hip(hipEventRecord(r->ev, s), "event record");
-        } catch (...) {
-            // part of the request may already be queued on s: its pool ranges
-            // are released by the caller only once that work has finished
-            (void)hipStreamSynchronize(s);
-            (void)hipGetLastError();
-            throw;
+        check_hip(fill_synthetic(in, reals, cfg_.precision == kF64, 0, r->r.seed, s), "fill");
+        r->plan->execute(in, r->plan->default_in_dist(r->r.direction), out,
+                         r->plan->default_out_dist(r->r.direction), r->r.batch, r->r.direction,
+                         s, ws, r->ws_bytes);
+        r->ev = get_event();
+        check_hip(hipEventRecord(r->ev, s), "event record");
+    }
+
+    // Newly appended descriptors reach the device table before any launch
+    // can use them (synchronously: this happens once per plan and direction).
+    template <typename T>
+    void sync_table(UnitTable<T> &table) {
+        const auto &d = table.descs();
+        if (d.size() > kMaxDescs) throw Error(SFFT_ERR_BUDGET, "serving descriptor table full");
+        if (d.size() == uploaded_descs_) return;
+        PassArgs<T> *h = static_cast<PassArgs<T> *>(h_table_);
+        for (size_t i = uploaded_descs_; i < d.size(); ++i) h[i] = d[i];
+        check_hip(hipMemcpy(static_cast<PassArgs<T> *>(d_table_) + uploaded_descs_,
+                            h + uploaded_descs_, (d.size() - uploaded_descs_) * sizeof(PassArgs<T>),
+                            hipMemcpyHostToDevice),
+                  "descriptor upload");
+        uploaded_descs_ = d.size();
+    }
+
+    // Small requests in one persistent launch (plus one fill launch) on st.
+    // Returns false if the work list does not fit (the caller runs them one
+    // by one instead).
+    template <typename T>
+    bool launch_merged(const std::vector<Req *> &rs, StreamCtx &st, UnitTable<T> &table) {
+        UnitBuilder<T> b;
+        for (Req *r : rs) {
+            // the table's descriptors hold the plan's device tables and are
+            // keyed by its address: the plan stays alive while the table does
+            table_plans_.insert(r->plan);
+            b.add(table, *r->plan, r->r.kind, r->r.direction, r->r.batch, r->in_off, r->out_off);
         }
-        if (cfg_.plain) check_hip(hipStreamSynchronize(s), "plain sync");
+        std::vector<WorkUnit> units;
+        std::vector<int64_t> gin, gout;
+        b.finish(units, gin, gout);
+        if (units.size() > kMaxUnits || gin.size() > kMaxGather || rs.size() > kMaxFills)
+            return false;
+        sync_table(table);
+        // the staging block is rewritten only after its last copy was read
+        if (st.copied) check_hip(hipEventSynchronize(st.copy_done), "staging reuse");
+        char *h = st.h_block;
+        const size_t ub = units.size() * sizeof(WorkUnit), gb = gin.size() * sizeof(int64_t);
+        std::memcpy(h, units.data(), ub);
+        std::memcpy(h + ub, gin.data(), gb);
+        std::memcpy(h + ub + gb, gout.data(), gb);
+        FillEntry *fe = reinterpret_cast<FillEntry *>(h + ub + 2 * gb);
+        const size_t fb = rs.size() * sizeof(FillEntry);
+        const int f64 = cfg_.precision == kF64;
+        for (size_t i = 0; i < rs.size(); ++i)
+            fe[i] = FillEntry{pool_->base() + rs[i]->in_off,
+                              (uint64_t)(rs[i]->in_bytes / (f64 ? 8 : 4)), rs[i]->r.seed, f64, 0};
+        check_hip(hipMemcpyAsync(st.d_block, h, ub + 2 * gb + fb, hipMemcpyHostToDevice, st.s),
+                  "work-list upload");
+        check_hip(hipEventRecord(st.copy_done, st.s), "event record");
+        st.copied = true;
+        const WorkUnit *du = reinterpret_cast<const WorkUnit *>(st.d_block);
+        const int64_t *dgi = reinterpret_cast<const int64_t *>(st.d_block + ub);
+        const int64_t *dgo = reinterpret_cast<const int64_t *>(st.d_block + ub + gb);
+        check_hip(fill_synthetic_many(reinterpret_cast<const FillEntry *>(st.d_block + ub + 2 * gb),
+                                      (int)rs.size(), st.s),
+                  "fill");
+        const unsigned grid = persistent_grid<T>(table.lds_need(), units.size());
+        check_hip(launch_persistent<T>(du, (int)units.size(),
+                                       static_cast<const PassArgs<T> *>(d_table_), dgi, dgo,
+                                       st.d_counters, pool_->base(), pool_->base(), grid,
+                                       table.lds_need(), st.s),
+                  "merged launch");
+        for (Req *r : rs) {
+            r->ev = get_event();
+            check_hip(hipEventRecord(r->ev, st.s), "event record");
+        }
+        return true;
+    }
+
+    // Enqueues the device work of the requests admitted in one pass (worker
+    // thread, mu_ not held).  Requests whose launch failed are returned in
+    // `failed` after their stream has drained.
+    void launch_group(std::vector<Req *> &rs, std::vector<std::pair<Req *, int>> &failed,
+                      std::vector<std::pair<Req *, StreamCtx *>> &launched) {
+        std::vector<Req *> merge, single;
+        for (Req *r : rs) {
+            const bool small = r->in_bytes + r->out_bytes <= kMergeBytes && r->ws_bytes == 0 &&
+                               !r->plan->bluestein();
+            (small && !cfg_.plain ? merge : single).push_back(r);
+        }
+        if (merge.size() < 2) {
+            single.insert(single.end(), merge.begin(), merge.end());
+            merge.clear();
+        }
+        if (!merge.empty()) {
+            StreamCtx &st = *streams_[next_stream_++ % streams_.size()];
+            int status = SFFT_OK;
+            bool ok = false;
+            try {
+                ok = cfg_.precision == kF64 ? launch_merged<double>(merge, st, *table64_)
+                                            : launch_merged<float>(merge, st, *table32_);
+            } catch (const Error &e) {
+                status = e.code;
+            } catch (const std::exception &) {
+                status = SFFT_ERR_DEVICE;
+            }
+            if (status != SFFT_OK) {
+                (void)hipStreamSynchronize(st.s);  // no queued work still uses the buffers
+                (void)hipGetLastError();
+                for (Req *r : merge) failed.push_back({r, status});
+            } else if (ok) {
+                for (Req *r : merge) launched.push_back({r, &st});
+            } else {
+                single.insert(single.end(), merge.begin(), merge.end());
+            }
+        }
+        for (Req *r : single) {
+            StreamCtx &st = *streams_[next_stream_++ % streams_.size()];
+            try {
+                launch_single(r, st.s);
+                if (cfg_.plain) check_hip(hipStreamSynchronize(st.s), "plain sync");
+                launched.push_back({r, &st});
+            } catch (const std::exception &e) {
+                // part of the request may already be queued: its pool ranges
+                // are released only once that work has finished
+                (void)hipStreamSynchronize(st.s);
+                (void)hipGetLastError();
+                const auto *err = dynamic_cast<const Error *>(&e);
+                failed.push_back({r, err ? err->code : SFFT_ERR_DEVICE});
+            }
+        }
+    }
+
+    bool try_admit(Req *r) {  // mu_ held
+        const size_t a = pool_->alloc(r->in_bytes);
+        if (a == SIZE_MAX) return false;
+        const size_t b = pool_->alloc(r->out_bytes);
+        if (b == SIZE_MAX) {
+            pool_->release(a, r->in_bytes);
+            return false;
+        }
+        size_t w = SIZE_MAX;
+        if (r->ws_bytes) {
+            w = pool_->alloc(r->ws_bytes);
+            if (w == SIZE_MAX) {
+                pool_->release(a, r->in_bytes);
+                pool_->release(b, r->out_bytes);
+                return false;
+            }
+        }
+        r->in_off = a;
+        r->out_off = b;
+        r->ws_off = w;
+        return true;
     }
 
     void loop() {
         std::unique_lock<std::mutex> lock(mu_);
+        std::vector<Req *> group;
+        std::vector<std::pair<Req *, int>> failed;
+        std::vector<std::pair<Req *, StreamCtx *>> launched;
         while (true) {
-            bool progress = false;
-            // 1. retire
-            for (size_t i = 0; i < inflight_.size();) {
-                Req *r = inflight_[i];
-                const hipError_t q = hipEventQuery(r->ev);
-                if (q == hipErrorNotReady) {
-                    ++i;
-                    continue;
-                }
-                r->t_done = now();
-                if (q != hipSuccess) {
-                    (void)hipGetLastError();
-                    r->status = SFFT_ERR_DEVICE;
-                }
-                pool_->release(r->in_off, r->in_bytes);
-                if (r->r.retain && r->status == SFFT_OK)
-                    retained_[r->r.id] = {r->out_off, r->out_bytes};
-                else
-                    pool_->release(r->out_off, r->out_bytes);
-                inflight_[i] = inflight_.back();
-                inflight_.pop_back();
-                complete(r);
-                progress = true;
-            }
-            // 2. admit in arrival order
-            while (!pending_.empty() && (int)inflight_.size() < cfg_.max_inflight) {
+            // admit in arrival order while slots and memory allow
+            group.clear();
+            while (!pending_.empty() && inflight_ + (int)group.size() < cfg_.max_inflight) {
                 Req *r = pending_.front();
-                const size_t a = pool_->alloc(r->in_bytes);
-                if (a == SIZE_MAX) break;
-                const size_t b = pool_->alloc(r->out_bytes);
-                if (b == SIZE_MAX) {
-                    pool_->release(a, r->in_bytes);
-                    break;
-                }
+                if (!try_admit(r)) break;
                 pending_.pop_front();
-                r->in_off = a;
-                r->out_off = b;
                 r->t_admit = now();
-                // The device work is enqueued (and, in plain mode, waited
-                // for) without the lock: launch() touches only this thread's
-                // state, so submit / poll / release are not held up.
-                int status = SFFT_OK;
+                group.push_back(r);
+            }
+            if (!group.empty()) {
+                inflight_ += (int)group.size();
+                // the device work is enqueued without the lock: submit / poll /
+                // release and the waiters are not held up
+                failed.clear();
+                launched.clear();
                 lock.unlock();
-                try {
-                    launch(r);
-                } catch (const Error &e) {
-                    status = e.code;
-                } catch (const std::exception &) {
-                    status = SFFT_ERR_DEVICE;
-                }
+                launch_group(group, failed, launched);
                 lock.lock();
-                if (status == SFFT_OK) {
-                    inflight_.push_back(r);
-                } else {
-                    pool_->release(a, r->in_bytes);
-                    pool_->release(b, r->out_bytes);
-                    r->status = status;
-                    r->t_done = now();
-                    complete(r);
+                for (auto &f : failed) {
+                    f.first->status = f.second;
+                    retire(f.first, hipSuccess);
                 }
-                progress = true;
+                for (auto &l : launched) {
+                    if (cfg_.plain) {
+                        retire(l.first, hipEventQuery(l.first->ev));
+                    } else {
+                        l.second->fifo.push_back(l.first);
+                        l.second->cv.notify_one();
+                    }
+                }
+                continue;
             }
-            if (stop_ && pending_.empty() && inflight_.empty()) break;
-            if (!progress) {
-                if (inflight_.empty() && pending_.empty())
-                    cv_.wait(lock);
-                else
-                    cv_.wait_for(lock, std::chrono::microseconds(20));
-            }
+            if (stop_ && pending_.empty() && inflight_ == 0) break;
+            cv_.wait(lock);
         }
     }
 
     sfft_server_config cfg_;
     std::unique_ptr<Pool> pool_;
-    std::vector<hipStream_t> streams_;
+    std::vector<std::unique_ptr<StreamCtx>> streams_;
     size_t next_stream_ = 0;
+    std::mutex ev_mu_;
     std::vector<hipEvent_t> event_pool_;
+    // merged launches: device descriptor table (append-only) and its mirror
+    void *d_table_ = nullptr, *h_table_ = nullptr;
+    size_t uploaded_descs_ = 0;
+    std::unique_ptr<UnitTable<float>> table32_;
+    std::unique_ptr<UnitTable<double>> table64_;
+    std::set<std::shared_ptr<Plan>> table_plans_;
     std::chrono::steady_clock::time_point t0_;
     std::mutex mu_;
     std::condition_variable cv_, idle_cv_;
     std::deque<Req *> pending_;
-    std::vector<Req *> inflight_;
+    int inflight_ = 0;
     std::vector<sfft_completion> done_;
     std::unordered_map<uint64_t, std::pair<size_t, size_t>> retained_;
     size_t outstanding_ = 0;
-    bool stop_ = false;
+    bool stop_ = false, stop_waiters_ = false;
     std::thread worker_;
 };
 

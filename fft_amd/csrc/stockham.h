@@ -381,14 +381,14 @@ struct Block {
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             C2<T> x = {(T)0, (T)0};
-                            if (act) x = St::load(in, base + (int64_t)(j + mfma_point(i, hh) * nb) * es);
+                            if (act) x = St::load(in, base + (int64_t)(j + mfma_in_point(i, hh) * nb) * es);
                             if (conj_in_) x.y = -x.y;
                             v[q][i] = x;
                         }
                     } else {
                         const C2<T> *ls = lds + slot * lds_stride_;
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) v[q][i] = ls[lds_idx(j + mfma_point(i, hh) * nb)];
+                        for (int i = 0; i < 8; ++i) v[q][i] = ls[lds_idx(j + mfma_in_point(i, hh) * nb)];
                     }
                 }
             }
@@ -402,7 +402,7 @@ struct Block {
                         const C2<T> *w = tw_ + st.tw_off + k * tw_group(16);
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
-                            const int r = mfma_point(i, hh);
+                            const int r = mfma_in_point(i, hh);
                             if (r) v[q][i] = cmul(v[q][i], w[r - 1]);
                         }
                     }

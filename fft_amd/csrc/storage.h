@@ -10,20 +10,20 @@
 // 16-point DFT of 32 columns is the real 32x32 product
 //     [Yr; Yi] = [[C, S], [-S, C]] [Xr; Xi],  C = cos(2 pi m c / 16),
 //                                              S = sin(2 pi m c / 16)
-// done as four v_mfma_f32_32x32x8 steps (K = 8 real rows each) accumulated
-// in float.  Operands are split into two 16-bit parts (x = hi + lo, both for
-// the data and for the DFT matrix) and the three products hi*hi, hi*lo,
-// lo*hi are accumulated, which keeps the contraction near float accuracy
-// (the lo*lo term is below 2^-21 of the result).  With IEEE half each
-// column is scaled by a power of two before the split so that half's range
-// is never exceeded; bfloat16 has float's exponent range and needs none.
+// done as two gfx950 v_mfma_f32_32x32x16 steps (K = 16 real rows each)
+// accumulated in float.  Operands are split into two 16-bit parts (x = hi +
+// lo, both for the data and for the DFT matrix) and the three products
+// hi*hi, hi*lo, lo*hi are accumulated, which keeps the contraction near float
+// accuracy (the lo*lo term is below 2^-21 of the result).  With IEEE half
+// each column is scaled by a power of two before the split so that half's
+// range is never exceeded; bfloat16 has float's exponent range and needs none.
 //
-// Lane layout of v_mfma_f32_32x32x8 (checked on gfx950): lane l holds
-// A[l % 32][4 (l / 32) + i] and B[4 (l / 32) + i][l % 32] for i < 4, and the
-// result D[8 (a / 4) + 4 (l / 32) + (a % 4)][l % 32] in accumulator a < 16.
-// With real rows 0..15 = re, 16..31 = im, lane l of column l % 32 therefore
-// holds points {4h..4h+3, 8+4h..8+4h+3} (h = l / 32) of its butterfly, both
-// as input and as output.
+// Lane layout of v_mfma_f32_32x32x16: lane l holds A[l % 32][8 (l / 32) + i]
+// and B[8 (l / 32) + i][l % 32] for i < 8, and the result
+// D[8 (a / 4) + 4 (l / 32) + (a % 4)][l % 32] in accumulator a < 16.  With
+// real rows 0..15 = re, 16..31 = im, lane l of column l % 32 (h = l / 32)
+// therefore takes input points 8h..8h+7 of its butterfly (mfma_in_point) and
+// holds output points {4h..4h+3, 8+4h..8+4h+3} (mfma_point).
 #pragma once
 
 #include <stdint.h>
@@ -115,7 +115,7 @@ struct Mfma16;
 
 template <>
 struct Mfma16<StoreF16> {
-    typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+    typedef _Float16 v8 __attribute__((ext_vector_type(8)));
     static constexpr bool kScale = true;  // half's range: scale each column
     __device__ static void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
         const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
@@ -123,14 +123,14 @@ struct Mfma16<StoreF16> {
         hi = __builtin_bit_cast(uint32_t, h);
         lo = __builtin_bit_cast(uint32_t, l);
     }
-    __device__ static f32x16 mma(v4 a, v4 b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, c, 0, 0, 0);
+    __device__ static f32x16 mma(v8 a, v8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     }
 };
 
 template <>
 struct Mfma16<StoreBF16> {
-    typedef short v4 __attribute__((ext_vector_type(4)));
+    typedef __bf16 v8 __attribute__((ext_vector_type(8)));
     typedef __bf16 b2 __attribute__((ext_vector_type(2)));
     static constexpr bool kScale = false;  // float's exponent range
     __device__ static void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
@@ -141,14 +141,17 @@ struct Mfma16<StoreBF16> {
         hi = hw;
         lo = __builtin_bit_cast(uint32_t, l);
     }
-    __device__ static f32x16 mma(v4 a, v4 b, f32x16 c) {
-        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0);
+    __device__ static f32x16 mma(v8 a, v8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
     }
 };
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <typename S>
-__device__ __forceinline__ typename Mfma16<S>::v4 as_v4(uint32_t w0, uint32_t w1) {
-    return __builtin_bit_cast(typename Mfma16<S>::v4, (u32x2){w0, w1});
+__device__ __forceinline__ typename Mfma16<S>::v8 as_v8(uint32_t w0, uint32_t w1, uint32_t w2,
+                                                       uint32_t w3) {
+    return __builtin_bit_cast(typename Mfma16<S>::v8, (u32x4){w0, w1, w2, w3});
 }
 
 // cos(2 pi t / 16), t = 0..15.
@@ -158,14 +161,16 @@ static __constant__ float kCos16[16] = {
     -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
     0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
 
-// Point index r of local value i (0..7) held by lane half h (0 or 1).
+// Output point index of local value i (0..7) held by lane half h (0 or 1).
 __device__ __forceinline__ int mfma_point(int i, int h) { return (i < 4) ? 4 * h + i : 8 + 4 * h + (i - 4); }
+// Input point index of local value i held by lane half h.
+__device__ __forceinline__ int mfma_in_point(int i, int h) { return 8 * h + i; }
 
 // The DFT-16 matrix operands of this lane, split hi + lo: for K step ks the
-// lane holds G[m][8 ks + 4 h + i], m = lane % 32, i < 4.
+// lane holds G[m][16 ks + 8 h + i], m = lane % 32, i < 8.
 template <typename S>
 struct Dft16Operands {
-    typename Mfma16<S>::v4 hi[4], lo[4];
+    typename Mfma16<S>::v8 hi[2], lo[2];
     __device__ static float entry(int m, int c) {
         const int mm = m & 15, cc = c & 15;
         const bool out_im = m >= 16, in_im = c >= 16;
@@ -176,23 +181,25 @@ struct Dft16Operands {
     __device__ explicit Dft16Operands(int lane) {
         const int m = lane & 31, h = lane >> 5;
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            const int c0 = 8 * ks + 4 * h;
-            uint32_t h0, l0, h1, l1;
-            Mfma16<S>::split2(entry(m, c0), entry(m, c0 + 1), h0, l0);
-            Mfma16<S>::split2(entry(m, c0 + 2), entry(m, c0 + 3), h1, l1);
-            hi[ks] = as_v4<S>(h0, h1);
-            lo[ks] = as_v4<S>(l0, l1);
+        for (int ks = 0; ks < 2; ++ks) {
+            const int c0 = 16 * ks + 8 * h;
+            uint32_t hw[4], lw[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                Mfma16<S>::split2(entry(m, c0 + 2 * p), entry(m, c0 + 2 * p + 1), hw[p], lw[p]);
+            hi[ks] = as_v8<S>(hw[0], hw[1], hw[2], hw[3]);
+            lo[ks] = as_v8<S>(lw[0], lw[1], lw[2], lw[3]);
         }
     }
 };
 
 // Forward 16-point DFTs of G groups of 32 columns held by a wave (8 points
-// per lane and group, local order of mfma_point), in place.  K steps: ks 0 =
-// re of local points 0..3, 1 = re of 4..7, 2 = im of 0..3, 3 = im of 4..7
-// (rows 8 ks + 4 h + i).  The MFMAs of the groups are interleaved, and the
-// hi*hi products and the two correction products accumulate separately, so
-// 2 G independent accumulation chains are in flight.
+// per lane and group: input points in the order of mfma_in_point, output
+// points in the order of mfma_point), in place.  K steps: ks 0 = re of the
+// lane's input points, 1 = their im (rows 16 ks + 8 h + i).  The MFMAs of
+// the groups are interleaved, and the hi*hi products and the two correction
+// products accumulate separately, so 2 G independent accumulation chains are
+// in flight.
 template <typename S, int G>
 __device__ __forceinline__ void dft16_mfma(C2<float> (&v)[G][8], const Dft16Operands<S> &g) {
     using M = Mfma16<S>;
@@ -211,21 +218,19 @@ __device__ __forceinline__ void dft16_mfma(C2<float> (&v)[G][8], const Dft16Oper
                 v[q][i] = {ldexpf(v[q][i].x, -ex[q]), ldexpf(v[q][i].y, -ex[q])};
         }
     }
-    typename M::v4 bh[G][4], bl[G][4];
+    typename M::v8 bh[G][2], bl[G][2];
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-        uint32_t h[4][2], l[4][2];
+        uint32_t h[2][4], l[2][4];
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < 4; ++p) {
             M::split2(v[q][2 * p].x, v[q][2 * p + 1].x, h[0][p], l[0][p]);
-            M::split2(v[q][4 + 2 * p].x, v[q][5 + 2 * p].x, h[1][p], l[1][p]);
-            M::split2(v[q][2 * p].y, v[q][2 * p + 1].y, h[2][p], l[2][p]);
-            M::split2(v[q][4 + 2 * p].y, v[q][5 + 2 * p].y, h[3][p], l[3][p]);
+            M::split2(v[q][2 * p].y, v[q][2 * p + 1].y, h[1][p], l[1][p]);
         }
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            bh[q][ks] = as_v4<S>(h[ks][0], h[ks][1]);
-            bl[q][ks] = as_v4<S>(l[ks][0], l[ks][1]);
+        for (int ks = 0; ks < 2; ++ks) {
+            bh[q][ks] = as_v8<S>(h[ks][0], h[ks][1], h[ks][2], h[ks][3]);
+            bl[q][ks] = as_v8<S>(l[ks][0], l[ks][1], l[ks][2], l[ks][3]);
         }
     }
     f32x16 acc[G], cor[G];
@@ -235,7 +240,7 @@ __device__ __forceinline__ void dft16_mfma(C2<float> (&v)[G][8], const Dft16Oper
         cor[q] = (f32x16){};
     }
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
+    for (int ks = 0; ks < 2; ++ks) {
 #pragma unroll
         for (int q = 0; q < G; ++q) {
             acc[q] = M::mma(g.hi[ks], bh[q][ks], acc[q]);

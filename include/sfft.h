@@ -233,7 +233,9 @@ typedef struct {
 } sfft_batch_item;
 
 /* Builds the work list and the per-size plans (no device work per item at
- * execute time beyond the one launch). */
+ * execute time beyond the one launch).  Items of chirp-z lengths (prime
+ * factors above 13) run as their plan's own launches after the persistent
+ * launch, in the same captured graph, through a workspace the batch owns. */
 int sfft_batch_create(sfft_batch *batch, const sfft_batch_item *items, size_t count,
                       int precision);
 int sfft_batch_destroy(sfft_batch batch);

@@ -164,12 +164,46 @@ def test_mixed_kinds_in_one_launch():
     assert not bad, bad
 
 
+def test_chirp_z_items_in_the_batch():
+    """Lengths with prime factors above 13 (chirp-z plans) in an uneven batch,
+    all kinds: bitwise equal to the per-size loop, and the oracle."""
+    sizes = [1009, 4096, 17 * 64, 2 * 4099, 1920, 3 * 1031, 65536, 2 * 997]
+    kinds = [L.C2C, L.C2C, L.C2C, L.R2C, L.C2C, L.C2C, L.C2C, L.R2C_HALFSHIFT]
+    dirs = [0, 1, 1, 0, 0, 1, 0, 1]
+    b, xin = _make(sizes, dirs, kinds, seed=23)
+    out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
+    ref = torch.empty_like(out)
+    for _ in range(2):  # graph capture, then a replay
+        out.zero_()
+        b.execute(xin, out)
+        torch.cuda.synchronize()
+    b.execute_per_size(xin, ref)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    for i, (n, k, d) in enumerate(zip(sizes, kinds, dirs)):
+        hs = k == L.R2C_HALFSHIFT
+        nb = 8 * n if k == L.C2C else 4 * n
+        ib = xin[b.in_offsets[i]:b.in_offsets[i] + nb]
+        ob = out[b.out_offsets[i]:b.out_offsets[i] + nb]
+        if k == L.C2C:
+            got = ob.view(torch.complex64).cpu().numpy()
+            want = po.fft(ib.view(torch.complex64).cpu().numpy().astype(np.complex128),
+                          inverse=bool(d))
+        elif d == 0:
+            got = ob.view(torch.complex64).cpu().numpy()
+            want = po.real_fft(ib.view(torch.float32).cpu().numpy().astype(np.float64),
+                               half_shift=hs)
+        else:
+            got = ob.view(torch.float32).cpu().numpy()
+            want = po.real_ifft(ib.view(torch.complex64).cpu().numpy().astype(np.complex128),
+                                half_shift=hs)
+        assert po.rel_l2(got, want) <= TOL32, (n, k, d)
+
+
 def test_uneven_batch_double():
     sizes, dirs = workloads.uneven_batch(seed=9, count=300)
     kinds = [L.C2C if i % 3 else (L.R2C if i % 2 else L.R2C_HALFSHIFT) for i in range(300)]
     sizes = [int(s) + (int(s) % 2 if k != L.C2C else 0) for s, k in zip(sizes, kinds)]
-    sizes = [s if fft_amd.Plan(s, k, L.F64).info()["chirp_length"] == 0 else 2 * 1024
-             for s, k in zip(sizes, kinds)]
     b = fft_amd.UnevenBatch(sizes, [int(d) for d in dirs], kinds, precision=L.F64)
     xin = torch.empty(b.in_bytes, dtype=torch.uint8, device="cuda")
     fft_amd.fill(xin.view(torch.float64), seed=9, dist="uniform")
@@ -194,4 +228,7 @@ def test_uneven_batch_double():
         else:
             want = po.real_ifft(ib.view(torch.complex128).cpu().numpy(), half_shift=hs)
             got = ob.view(torch.float64).cpu().numpy()
-        assert po.rel_l2(got, want) <= 1e-12, (n, k, d)
+        # chirp-z lengths (real kinds whose half length has a prime factor
+        # above 13) differ from the oracle's generic radix by its phase rounding
+        chirp = fft_amd.Plan(n, k, L.F64).info()["chirp_length"] != 0
+        assert po.rel_l2(got, want) <= (1e-9 if chirp else 1e-12), (n, k, d)

@@ -99,6 +99,33 @@ def test_poisson_run_full_range():
     assert s["p50_ms"] > 0 and s["p99_ms"] >= s["p50_ms"]
 
 
+def test_burst_of_small_requests_merged():
+    """A burst of small requests (admitted together, so they run in merged
+    persistent launches) of every kind and both directions, retained: each
+    equals the plain plan on the same input, bitwise."""
+    rng = np.random.default_rng(4)
+    sizes = [256, 300, 512, 1000, 1024, 1920, 2048, 3000, 4096, 6000, 8192, 16384, 20000,
+             32768, 65536]
+    reqs = []
+    for i in range(120):
+        n = int(rng.choice(sizes))
+        kind = [L.C2C, L.C2C, L.R2C][i % 3]
+        d = int(rng.integers(0, 2))
+        b = int(rng.integers(1, 9))
+        reqs.append((n, kind, d, b))
+    with Server(budget_bytes=4 << 30, max_inflight=64, streams=4) as srv:
+        for i, (n, k, d, b) in enumerate(reqs):
+            srv.submit(i, n, k, d, b, seed=5000 + i, retain=True)
+        srv.drain()
+        rec = srv.poll()
+        assert len(rec) == len(reqs) and (rec["status"] == 0).all()
+        for i, (n, k, d, b) in enumerate(reqs):
+            x, y = _alone(n, k, d, b, 5000 + i)
+            got = srv.fetch(i, y.nbytes).view(y.dtype).reshape(y.shape)
+            srv.release(i)
+            assert np.array_equal(got, y), (i, n, k, d, b)
+
+
 def test_backpressure_under_small_budget():
     # 32 requests of 32 MiB each through a 96 MiB pool: at most a few fit at once
     with Server(budget_bytes=96 << 20, max_inflight=16) as srv:
