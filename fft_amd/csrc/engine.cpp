@@ -158,16 +158,6 @@ static int max_block(int prec) {
     return stored_16(prec) ? kMaxBlock16 : (arith_f64(prec) ? kMaxBlock<double>() : kMaxBlock<float>());
 }
 
-// Bytes of two-pass intermediate per chunk of a batch (0: no chunking):
-// 64 MiB, or SFFT_TWO_PASS_CHUNK_BYTES.
-size_t two_pass_chunk_bytes() {
-    static const size_t v = [] {
-        const char *e = std::getenv("SFFT_TWO_PASS_CHUNK_BYTES");
-        return e && *e ? (size_t)std::strtoull(e, nullptr, 10) : (size_t)64 << 20;
-    }();
-    return v;
-}
-
 // Dynamic LDS a layout may use: the device limit less room for the kernels'
 // static __shared__ variables.
 static size_t lds_budget() { return device_info().lds_bytes - kLdsReserve; }
@@ -660,18 +650,7 @@ void Plan::run(const void *in, size_t in_dist, void *out, size_t out_dist, size_
         run_bluestein<T>(in, din, out, dout, batch, inverse, stream, ws);
         return;
     }
-    // Two-pass batches run in chunks whose intermediate (written by the first
-    // pass, read back by the second) fits the last-level cache, so the second
-    // pass reads it from there rather than from HBM.
-    const size_t np = passes_.size();
-    const size_t cb = 2 * real_bytes(prec_);  // one complex value in HBM
-    size_t chunk = batch;
-    if (np > 1 && two_pass_chunk_bytes()) chunk = std::max<size_t>(1, two_pass_chunk_bytes() / (cn_ * cb));
-    for (size_t c0 = 0; c0 < batch; c0 += chunk) {
-        const size_t n = std::min(chunk, batch - c0);
-        run_passes<T>((const char *)in + c0 * (size_t)din * cb, din, (char *)out + c0 * (size_t)dout * cb,
-                      dout, n, inverse, stream, ws ? (char *)ws + c0 * cn_ * cb : nullptr);
-    }
+    run_passes<T>(in, din, out, dout, batch, inverse, stream, ws);
 }
 
 // The passes of a direct plan over one batch (distances in complex values).

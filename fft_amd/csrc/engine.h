@@ -186,8 +186,6 @@ std::shared_ptr<Plan> get_plan(size_t n, int kind, int prec);
 bool choose_radices(int n, int ept, std::vector<int> &radices, int max_radix = 25,
                     int max_threads = 1024);
 bool supported_length(size_t n);
-// Intermediate bytes per chunk of a two-pass batch (0: the whole batch at once).
-size_t two_pass_chunk_bytes();
 // Any length a plan exists for in F32 / F64: direct, or chirp-z (up to 2^27).
 bool plannable_length(size_t n);
 

@@ -251,9 +251,14 @@ int sfft_batch_execute_per_size(sfft_batch batch, const void *in_arena, void *ou
 /* ---- serving: continuous batching of transform requests ----
  * No counterpart in the reference (a single-threaded library).  Requests of
  * any kind, size and batch are admitted in arrival order as in-flight slots
- * and HBM budget free up; admitted requests run concurrently on several
- * device streams.  Inputs are synthesised in HBM from the request's seed
- * (stand-in for the client's upload); outputs stay in the server's pool. */
+ * and HBM budget (inputs, outputs and chirp-z workspaces) free up.  The small
+ * requests admitted together (up to 16 MiB of input + output each) run merged
+ * in one persistent launch; larger ones run their plan's own kernels; all of
+ * them overlap on several device streams, and a waiter per stream retires
+ * them as their completion events fire.  Each result equals the plain
+ * sfft_execute of the same request bitwise.  Inputs are synthesised in HBM
+ * from the request's seed (stand-in for the client's upload); outputs stay in
+ * the server's pool. */
 
 typedef struct sfft_server_s *sfft_server;
 

@@ -227,17 +227,16 @@ def test_two_pass_sizes(n, batch):
     assert po.rel_l2(z, po.ifft(y.astype(np.complex128))) <= TOL32
 
 
-def test_two_pass_chunked_batch():
-    """A two-pass batch larger than one last-level-cache chunk (2^20 x 12 =
-    96 MiB of intermediate, run in chunks of 64 MiB): every transform equals
-    the same transform run alone, bitwise, and the oracle."""
+def test_two_pass_batch_matches_alone():
+    """A two-pass batch (2^20 x 12): every transform equals the same
+    transform run alone, bitwise, and the oracle."""
     n, batch = 1 << 20, 12
     x = torch.empty(batch, n, dtype=torch.complex64, device="cuda")
     fft_amd.fill(x, seed=77, dist="normal")
     y = torch.empty_like(x)
     f = fft_amd.FFT(n)
     f.fft(x, y)
-    for i in (0, 7, 8, 11):  # both chunks, their edges
+    for i in (0, 7, 8, 11):
         yi = torch.empty_like(x[i:i + 1])
         f.fft(x[i:i + 1], yi)
         torch.cuda.synchronize()
