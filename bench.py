@@ -191,6 +191,45 @@ def plain_per_call(torch, fft_amd, L, bufs, sample):
     return total_bytes / total_ms / 1e6, per
 
 
+def fast_sizes_batched(torch, fft_amd, lo=64, hi=65536, target_bytes=256 << 20, reps=3):
+    """Every fast size 2^a 3^b 5^c in [lo, hi] as one batched FFT<float>
+    forward launch of about target_bytes of input (inputs the last-level
+    cache does not hold: two buffer sets alternate): the median and the
+    spread of GB/s over the sizes, and which kernel family ran them."""
+    from fft_amd import _lib as L
+    from fft_amd import workloads
+    sh = torch.cuda.current_stream().cuda_stream
+    sizes = workloads.fast_sizes(lo, hi)
+    cap = target_bytes // 8
+    bufs = [torch.empty(cap, dtype=torch.complex64, device="cuda") for _ in range(4)]
+    for t in bufs:
+        fft_amd.fill(t, seed=11)
+    rates, compiled = [], 0
+    for n in sizes:
+        batch = max(1, cap // n)
+        p = fft_amd.Plan(n, L.C2C, L.F32)
+        compiled += L.lib().sfft_plan_variants(n, L.C2C, L.F32) > 0
+        for i in range(2):
+            p.execute_ptr(bufs[i].data_ptr(), bufs[2 + i].data_ptr(), batch, 0, sh)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for r in range(2 * reps):
+            p.execute_ptr(bufs[r % 2].data_ptr(), bufs[2 + r % 2].data_ptr(), batch, 0, sh)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / (2 * reps)
+        rates.append(16 * n * batch / ms / 1e6)
+    del bufs
+    torch.cuda.empty_cache()
+    r = np.array(rates)
+    return {"sizes": len(sizes), "compiled_kernels": int(compiled),
+            "median_GBs": round(float(np.median(r)), 1),
+            "p10_GBs": round(float(np.percentile(r, 10)), 1),
+            "p90_GBs": round(float(np.percentile(r, 90)), 1),
+            "min_GBs": round(float(r.min()), 1), "max_GBs": round(float(r.max()), 1),
+            "bytes_per_launch": target_bytes}
+
+
 def uneven_batch(torch, fft_amd, count=20000, reps=5):
     """The uneven batch: 20000 transforms, fast sizes log-uniform in [64, 65536],
     direction 50/50, U[-1,1) inputs; one persistent graph-captured launch with
@@ -525,6 +564,7 @@ def main():
     }
     if not args.no_extras:
         result["single_transform_4096"] = single_transform(torch, fft_amd)
+        result["fast_sizes_batched"] = fast_sizes_batched(torch, fft_amd)
         result["uneven_batch"] = uneven_batch(torch, fft_amd)
         result["precision_sweep"] = precision_sweep(torch, fft_amd)
         result["serving"] = serving()
