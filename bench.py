@@ -46,6 +46,9 @@ def parse():
                     help="transforms per size timed on the plain per-call path (0: skip)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the uneven-batch and serving measurements")
+    ap.add_argument("--uneven-only", action="store_true",
+                    help="only run the uneven batch's persistent launch (5 launches after one "
+                         "warm-up; for profiling its kernel) and print its timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only time a 1 GiB device-to-device copy (the reference kernel for "
                          "scaling HBM byte counters) and exit")
@@ -230,7 +233,7 @@ def fast_sizes_batched(torch, fft_amd, lo=64, hi=65536, target_bytes=256 << 20, 
             "bytes_per_launch": target_bytes}
 
 
-def uneven_batch(torch, fft_amd, count=20000, reps=5):
+def uneven_batch(torch, fft_amd, count=20000, reps=5, tune=True):
     """The uneven batch: 20000 transforms, fast sizes log-uniform in [64, 65536],
     direction 50/50, U[-1,1) inputs; one persistent graph-captured launch with
     the planner's plans, then with autotuned plans (every distinct size tuned
@@ -245,14 +248,18 @@ def uneven_batch(torch, fft_amd, count=20000, reps=5):
     out = torch.empty(b.out_bytes, dtype=torch.uint8, device="cuda")
     alg = b.in_bytes + b.out_bytes
     res = {}
-    t0 = time.perf_counter()
-    uniq = sorted(set(int(x) for x in sizes))
-    for n in uniq:
-        fft_amd.autotune(n, L.C2C, L.F32, max(1, (128 << 20) // (16 * n)))
-    tune_s = time.perf_counter() - t0
-    bt = fft_amd.UnevenBatch([int(x) for x in sizes], [int(d) for d in dirs])
-    for name, fn in (("persistent", b.execute), ("persistent_tuned", bt.execute),
-                     ("per_size_loop", b.execute_per_size)):
+    if not tune:
+        legs = (("persistent", b.execute),)
+    else:
+        t0 = time.perf_counter()
+        uniq = sorted(set(int(x) for x in sizes))
+        for n in uniq:
+            fft_amd.autotune(n, L.C2C, L.F32, max(1, (128 << 20) // (16 * n)))
+        tune_s = time.perf_counter() - t0
+        bt = fft_amd.UnevenBatch([int(x) for x in sizes], [int(d) for d in dirs])
+        legs = (("persistent", b.execute), ("persistent_tuned", bt.execute),
+                ("per_size_loop", b.execute_per_size))
+    for name, fn in legs:
         fn(xin, out)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -263,6 +270,8 @@ def uneven_batch(torch, fft_amd, count=20000, reps=5):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
         res[name] = {"ms": round(ms, 3), "GB/s": round(alg / ms / 1e6, 1)}
+    if not tune:
+        return res
     res["speedup"] = round(res["per_size_loop"]["ms"] / res["persistent"]["ms"], 2)
     res["speedup_tuned"] = round(res["per_size_loop"]["ms"] / res["persistent_tuned"]["ms"], 2)
     res["autotune"] = {"sizes": len(uniq), "seconds": round(tune_s, 2)}
@@ -501,6 +510,9 @@ def main():
     torch.cuda.set_device(0)
     if args.copy_calibration:
         copy_calibration(torch)
+        return
+    if args.uneven_only:
+        print(json.dumps(uneven_batch(torch, fft_amd, tune=False)))
         return
     launches, bufs = build_workload(torch, fft_amd, L)
     total_ms, wall = time_steps(torch, launches, args.steps, args.warmup)
