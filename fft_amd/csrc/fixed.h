@@ -319,21 +319,32 @@ struct Fixed {
             const uint32_t sbase = (t * (uint32_t)a.out_bs + c * (uint32_t)a.out_ss) * DB;
             const uint32_t mask = (1u << a.ptw_shift) - 1u;
 #pragma unroll
+            for (int b = 0; b < B; ++b)
+#pragma unroll
+                for (int t2 = 0; t2 < R; ++t2) flip_im(v[b][t2], a.flip_out);
+            if (a.post_tw) {
+                // the four-step factors of every element, loaded together
+                // (lanes without a butterfly read entry 0; never stored)
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = (active && jvalid(b)) ? lid + b * TPT : 0;
+                    const int o = (j / NS) * NS * R + j % NS;
+#pragma unroll
+                    for (int t2 = 0; t2 < R; ++t2) {
+                        const uint32_t m = active ? c * (uint32_t)(o + t2 * NS) : 0u;
+                        v[b][t2] = cmul(v[b][t2],
+                                        cmul(a.ptw_fine[m & mask], a.ptw_coarse[m >> a.ptw_shift]));
+                    }
+                }
+            }
+#pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = lid + b * TPT;
                 if (active && jvalid(b)) {
                     const int o = (j / NS) * NS * R + j % NS;
 #pragma unroll
-                    for (int t2 = 0; t2 < R; ++t2) {
-                        const uint32_t e = (uint32_t)(o + t2 * NS);
-                        C2<T> y = v[b][t2];
-                        flip_im(y, a.flip_out);
-                        if (a.post_tw) {
-                            const uint32_t m = c * e;
-                            y = cmul(y, cmul(a.ptw_fine[m & mask], a.ptw_coarse[m >> a.ptw_shift]));
-                        }
-                        st(dst + (sbase + e * es), y);
-                    }
+                    for (int t2 = 0; t2 < R; ++t2)
+                        st(dst + (sbase + (uint32_t)(o + t2 * NS) * es), v[b][t2]);
                 }
             }
         } else if constexpr (DST_G) {

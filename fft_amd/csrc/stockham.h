@@ -206,7 +206,15 @@ struct Block {
         return cmul(ptw_fine_[m & ((1u << post_tw_shift_) - 1u)], ptw_coarse_[m >> post_tw_shift_]);
     }
 
-    // One Stockham stage of radix R.
+    // One Stockham stage of radix R.  No branch separates a load from the
+    // next one: the data loads of all butterflies are issued before any value
+    // is used, and the table loads (twiddles, rotations, four-step factors)
+    // are unconditional, with the indices of lanes without a butterfly
+    // clamped to valid entries (their values are never stored), so the
+    // compiler keeps them in flight together.  (A load followed by its use
+    // and a branch on the pass mode waits out one latency per element.)  The
+    // arithmetic per element is that of stage_wide, so the results are
+    // bitwise the same.
     template <int R>
     __device__ __forceinline__ void stage(int s, bool src_global, bool dst_global, bool col_map) {
         if constexpr (R == 16 && !St::native && MFMA) {
@@ -215,6 +223,183 @@ struct Block {
                 return;
             }
         }
+        if constexpr (EPT >= 32) {
+            stage_wide<R>(s, src_global, dst_global, col_map);
+            return;
+        }
+        const StageDesc st = a.pd.st[s];
+        constexpr int B = (EPT / R) > 0 ? (EPT / R) : 1;
+        const int n = n_;
+        const int nb = n / R;
+        const int ns = st.ns;
+        int slot, lid;
+        if (col_map) {
+            slot = tid % tpw_;
+            lid = tid / tpw_;
+        } else {
+            slot = tid / tpt_;
+            lid = tid % tpt_;
+        }
+        const int g = slot0 + slot;
+        const bool active = (slot < tpw_) && (lid < tpt_) && (g < total_slots);
+        C2<T> *ls = lds + slot * lds_stride_;
+        // butterflies of this thread: j = lid + b tpt, valid when ok(b)
+        auto ok = [&](int b) { return active && (lid + b * tpt_ < nb); };
+
+        C2<T> v[B][R];
+        // ---- load
+        if (St::native && src_global && mode_ == kModeC2RLoad) {
+            // two-pass inverse real: pre-process each element from its pair
+            int64_t tb = 0, cofs = 0;
+            if (active) split_slot(iin_, g, tb, cofs);
+            const int64_t es = iin_.elem_stride;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lid + b * tpt_;
+                if (ok(b)) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const int m = (int)(cofs + (int64_t)(j + r * nb) * es);
+                        if constexpr (St::native)
+                            v[b][r] = cconj(real_pre_bin(in + tb, real_tw_, m, real_h_,
+                                                         half_shift_ != 0));
+                    }
+                }
+            }
+        } else if (src_global) {
+            const int64_t base = active ? ibase(g) : 0;
+            const int64_t es = iin_.elem_stride;
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lid + b * tpt_;
+                if (ok(b)) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        v[b][r] = St::load(in, base + (int64_t)(j + r * nb) * es);
+                }
+            }
+            if (conj_in_) {
+#pragma unroll
+                for (int b = 0; b < B; ++b)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[b][r].y = -v[b][r].y;
+            }
+            if (half_shift_ && (mode_ == kModeR2C || mode_ == kModeR2CLoad)) {
+                // half-shift pre-rotation by element e of the transform (one
+                // pass), or by its index in the whole transform (first of two)
+                int64_t tb = 0, cofs = 0;
+                if (active && mode_ == kModeR2CLoad) split_slot(iin_, g, tb, cofs);
+                const int64_t rs = mode_ == kModeR2CLoad ? es : 1;
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = ok(b) ? lid + b * tpt_ : 0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        v[b][r] = cmul(v[b][r], rot_[cofs + (int64_t)(j + r * nb) * rs]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lid + b * tpt_;
+                if (ok(b)) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[b][r] = ls[lds_idx(j + r * nb)];
+                }
+            }
+            __syncthreads();  // every slot read before any slot is overwritten
+        }
+        // ---- twiddle + butterfly
+        if (ns > 1) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = ok(b) ? lid + b * tpt_ : 0;
+                const uint32_t q = fdiv((uint32_t)j, st.div);
+                const int k = j - (int)q * ns;
+                const C2<T> *wp = tw_ + st.tw_off + k * tw_group(R);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], wp[r - 1]);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) dft<R, T>(v[b]);
+        // ---- store
+        if (dst_global) {
+            const int64_t base = active ? obase(g) : 0;
+            const int64_t es = iout_.elem_stride;
+            if (conj_out_) {
+#pragma unroll
+                for (int b = 0; b < B; ++b)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[b][r].y = -v[b][r].y;
+            }
+            if (half_shift_ && (mode_ == kModeC2R || mode_ == kModeC2RStore)) {
+                int64_t tbo = 0, cofo = 0;
+                if (active && mode_ == kModeC2RStore) split_slot(iout_, g, tbo, cofo);
+                const int64_t rs = mode_ == kModeC2RStore ? es : 1;
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = ok(b) ? lid + b * tpt_ : 0;
+                    const uint32_t q = fdiv((uint32_t)j, st.div);
+                    const int o = (int)q * ns * R + (j - (int)q * ns);
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        v[b][r] = cmulc(v[b][r], rot_[cofo + (int64_t)(o + r * ns) * rs]);
+                }
+            }
+            if (post_tw_) {
+                // four-step twiddle W_N^{c e} (c: column of slot g) as the
+                // product of a fine and a coarse table entry
+                const uint32_t c =
+                    active ? (uint32_t)g - fdiv((uint32_t)g, iout_.per_batch) * iout_.per_batch.d : 0u;
+                const uint32_t mask = (1u << post_tw_shift_) - 1u;
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = ok(b) ? lid + b * tpt_ : 0;
+                    const uint32_t q = fdiv((uint32_t)j, st.div);
+                    const int o = (int)q * ns * R + (j - (int)q * ns);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        // c < N2 and e < N1, so c e < N: no reduction needed
+                        const uint32_t m = c * (uint32_t)(o + r * ns);
+                        v[b][r] = cmul(v[b][r],
+                                       cmul(ptw_fine_[m & mask], ptw_coarse_[m >> post_tw_shift_]));
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lid + b * tpt_;
+                if (ok(b)) {
+                    const uint32_t q = fdiv((uint32_t)j, st.div);
+                    const int o = (int)q * ns * R + (j - (int)q * ns);
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        St::store(out, base + (int64_t)(o + r * ns) * es, v[b][r]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lid + b * tpt_;
+                if (ok(b)) {
+                    const uint32_t q = fdiv((uint32_t)j, st.div);
+                    const int k = j - (int)q * ns;
+                    const int o = (int)q * ns * R + k;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) ls[lds_idx(o + r * ns)] = v[b][r];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // The stage of the 32-element instances: one element's accesses at a
+    // time (the grouped form of stage() needs more than the 128 registers a
+    // 1024-thread block allows at 32 elements per thread).  Same arithmetic.
+    template <int R>
+    __device__ __forceinline__ void stage_wide(int s, bool src_global, bool dst_global,
+                                               bool col_map) {
         const StageDesc st = a.pd.st[s];
         constexpr int B = (EPT / R) > 0 ? (EPT / R) : 1;
         const int n = n_;
