@@ -24,16 +24,21 @@ namespace sfft {
 template <typename T>
 class UnitTable {
   public:
-    explicit UnitTable(size_t lds_cap) : lds_cap_(lds_cap) {}
+    // lds_bytes: LDS of one CU
+    explicit UnitTable(size_t lds_bytes) : lds_bytes_(lds_bytes) {}
     // First descriptor index of (plan, direction), appending its passes.
     int entry(const Plan &p, int inverse);
     const std::vector<PassArgs<T>> &descs() const { return descs_; }
-    size_t lds_need() const { return lds_need_; }
+    // Persistent kernel class (persistent_class) of an entry's units, and the
+    // LDS the units of each class need.
+    int class_of(int e0) const { return cls_.at((size_t)e0); }
+    size_t lds_need(int cls) const { return lds_need_[cls]; }
 
   private:
-    size_t lds_cap_;
-    size_t lds_need_ = 0;
+    size_t lds_bytes_;
+    size_t lds_need_[kPersistentClasses] = {};
     std::vector<PassArgs<T>> descs_;
+    std::vector<int> cls_;  // per descriptor: its entry's class
     std::map<std::pair<const Plan *, int>, int> first_;
 };
 
@@ -44,20 +49,23 @@ class UnitBuilder {
     // offsets in_off / out_off of the arenas, the rest back to back.
     void add(UnitTable<T> &table, const Plan &p, int kind, int inverse, size_t count,
              size_t in_off, size_t out_off);
-    // Units longest first plus the gather lists (element offsets).
+    // Units plus the gather lists (element offsets), grouped by kernel class
+    // in class order (per_class[c] units of class c), longest first in each.
     void finish(std::vector<WorkUnit> &units, std::vector<int64_t> &gin,
-                std::vector<int64_t> &gout);
+                std::vector<int64_t> &gout, size_t (&per_class)[kPersistentClasses]);
     bool empty() const { return hu_.empty() && groups_.empty(); }
 
   private:
     struct HostUnit {
         WorkUnit u;
         double cost;
+        int cls;
     };
     std::vector<HostUnit> hu_;
     std::map<int, std::vector<std::pair<int64_t, int64_t>>> groups_;
     std::map<int, double> group_cost_;
     std::map<int, int> group_tpw_;
+    std::map<int, int> group_cls_;
 };
 
 class UnevenBatch {
@@ -77,12 +85,10 @@ class UnevenBatch {
     // The plain per-size loop: one batched launch sequence per item.
     void execute_per_size(const void *in_arena, void *out_arena, hipStream_t stream);
 
-    size_t units() const { return nunits_; }
-    unsigned grid() const { return grid_; }
 
   private:
     template <typename T>
-    void build(size_t lds_cap);
+    void build(size_t lds_bytes);
     void launch(const void *in_arena, void *out_arena, hipStream_t stream);
     void release();
 
@@ -91,12 +97,13 @@ class UnevenBatch {
     std::vector<std::shared_ptr<Plan>> plans_;
     WorkUnit *d_units_ = nullptr;
     void *d_table_ = nullptr;
-    unsigned *d_counters_ = nullptr;
+    unsigned *d_counters_ = nullptr;  // one queue (2 counters) per kernel class
     int64_t *d_gather_in_ = nullptr;
     int64_t *d_gather_out_ = nullptr;
-    size_t nunits_ = 0;
-    unsigned grid_ = 0;
-    size_t lds_bytes_ = 0;
+    // units grouped by kernel class (nunits_[c] of class c); one launch each
+    size_t nunits_[kPersistentClasses] = {};
+    unsigned grid_[kPersistentClasses] = {};
+    size_t lds_bytes_[kPersistentClasses] = {};
     // workspace of the chirp-z items (run after the persistent launch)
     void *d_ws_ = nullptr;
     size_t ws_bytes_ = 0;
@@ -111,8 +118,9 @@ class UnevenBatch {
     void *graph_out_ = nullptr;
 };
 
-// Workgroups of a persistent launch whose descriptors need lds_bytes of LDS.
+// Workgroups of a persistent launch of class cls whose descriptors need
+// lds_bytes of LDS.
 template <typename T>
-unsigned persistent_grid(size_t lds_bytes, size_t nunits);
+unsigned persistent_grid(int cls, size_t lds_bytes, size_t nunits);
 
 }  // namespace sfft

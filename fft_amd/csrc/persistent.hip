@@ -3,8 +3,9 @@
 
 namespace sfft {
 
-template <typename T, int EPT>
-__global__ void __launch_bounds__(persistent_block<T>())
+// MINW: waves per SIMD the register allocation must allow (4: 128 registers).
+template <typename T, int EPT, int THREADS, int MINW>
+__global__ void __launch_bounds__(THREADS, MINW)
     k_persistent(const WorkUnit *__restrict__ units, int nunits,
                  const PassArgs<T> *__restrict__ table, const int64_t *__restrict__ gather_in,
                  const int64_t *__restrict__ gather_out, unsigned *counters,
@@ -65,45 +66,79 @@ __global__ void __launch_bounds__(persistent_block<T>())
     }
 }
 
-template <typename T>
-static constexpr int persistent_ept() {
-    return sizeof(T) == 4 ? 32 : 16;
-}
-
-template <typename T>
-hipError_t launch_persistent(const WorkUnit *units, int nunits, const PassArgs<T> *table,
+template <typename T, int E, int TH, int MW>
+static hipError_t launch_one(const WorkUnit *units, int nunits, const PassArgs<T> *table,
                              const int64_t *gather_in, const int64_t *gather_out,
                              unsigned *counters, const void *in_arena, void *out_arena,
                              unsigned grid, size_t lds_bytes, hipStream_t stream) {
-    constexpr int E = persistent_ept<T>();
     static std::once_flag lds_once;
     static hipError_t lds_err = hipSuccess;
-    if (allow_max_lds(lds_once, lds_err, (const void *)k_persistent<T, E>) != hipSuccess) return lds_err;
-    hipLaunchKernelGGL((k_persistent<T, E>), dim3(grid), dim3(persistent_block<T>()), lds_bytes,
-                       stream, units, nunits, table, gather_in, gather_out, counters,
-                       (const char *)in_arena,
-                       (char *)out_arena);
+    if (allow_max_lds(lds_once, lds_err, (const void *)k_persistent<T, E, TH, MW>) != hipSuccess)
+        return lds_err;
+    hipLaunchKernelGGL((k_persistent<T, E, TH, MW>), dim3(grid), dim3(TH), lds_bytes, stream,
+                       units, nunits, table, gather_in, gather_out, counters,
+                       (const char *)in_arena, (char *)out_arena);
     return hipGetLastError();
 }
 
-template <typename T>
-int persistent_blocks_per_cu(size_t lds_bytes) {
-    constexpr int E = persistent_ept<T>();
+template <typename T, int E, int TH, int MW>
+static int occupancy(size_t lds_bytes) {
+    static std::once_flag lds_once;
+    static hipError_t lds_err = hipSuccess;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)k_persistent<T, E>,
-                                                     persistent_block<T>(), lds_bytes) != hipSuccess)
+    if (allow_max_lds(lds_once, lds_err, (const void *)k_persistent<T, E, TH, MW>) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)k_persistent<T, E, TH, MW>,
+                                                     TH, lds_bytes) != hipSuccess) {
+        (void)hipGetLastError();
         return 1;
+    }
     return blocks > 0 ? blocks : 1;
 }
 
-template hipError_t launch_persistent<float>(const WorkUnit *, int, const PassArgs<float> *,
-                                             const int64_t *, const int64_t *, unsigned *,
-                                             const void *, void *, unsigned, size_t, hipStream_t);
-template hipError_t launch_persistent<double>(const WorkUnit *, int, const PassArgs<double> *,
-                                              const int64_t *, const int64_t *, unsigned *,
-                                              const void *, void *, unsigned, size_t,
-                                              hipStream_t);
-template int persistent_blocks_per_cu<float>(size_t);
-template int persistent_blocks_per_cu<double>(size_t);
+// the instance of each float class (persistent_class): elements per thread,
+// threads, waves per SIMD
+template <>
+hipError_t launch_persistent<float>(int cls, const WorkUnit *units, int nunits,
+                                    const PassArgs<float> *table, const int64_t *gather_in,
+                                    const int64_t *gather_out, unsigned *counters,
+                                    const void *in_arena, void *out_arena, unsigned grid,
+                                    size_t lds_bytes, hipStream_t stream) {
+    switch (cls) {
+#define SFFT_CASE(C, E, TH)                                                                     \
+    case C:                                                                                   \
+        return launch_one<float, E, TH, 4>(units, nunits, table, gather_in, gather_out, counters, \
+                                           in_arena, out_arena, grid, lds_bytes, stream);
+        SFFT_CASE(0, 16, 512)
+        SFFT_CASE(1, 16, 1024)
+        SFFT_CASE(2, 32, 1024)
+#undef SFFT_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+template <>
+hipError_t launch_persistent<double>(int cls, const WorkUnit *units, int nunits,
+                                     const PassArgs<double> *table, const int64_t *gather_in,
+                                     const int64_t *gather_out, unsigned *counters,
+                                     const void *in_arena, void *out_arena, unsigned grid,
+                                     size_t lds_bytes, hipStream_t stream) {
+    if (cls != 0) return hipErrorInvalidValue;
+    return launch_one<double, 16, 512, 2>(units, nunits, table, gather_in, gather_out, counters,
+                                          in_arena, out_arena, grid, lds_bytes, stream);
+}
+
+template <>
+int persistent_blocks_per_cu<float>(int cls, size_t lds_bytes) {
+    switch (cls) {
+    case 0: return occupancy<float, 16, 512, 4>(lds_bytes);
+    case 1: return occupancy<float, 16, 1024, 4>(lds_bytes);
+    default: return occupancy<float, 32, 1024, 4>(lds_bytes);
+    }
+}
+template <>
+int persistent_blocks_per_cu<double>(int, size_t lds_bytes) {
+    return occupancy<double, 16, 512, 2>(lds_bytes);
+}
 
 }  // namespace sfft

@@ -251,14 +251,16 @@ class Server {
             StreamCtx &st = *streams_.back();
             check_hip(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking), "stream create");
             if (cfg_.plain) continue;
-            if (hipMalloc((void **)&st.d_counters, 2 * sizeof(unsigned)) != hipSuccess ||
+            if (hipMalloc((void **)&st.d_counters, 2 * kPersistentClasses * sizeof(unsigned)) !=
+                    hipSuccess ||
                 hipMalloc((void **)&st.d_block, kBlockBytes) != hipSuccess ||
                 hipHostMalloc((void **)&st.h_block, kBlockBytes, hipHostMallocDefault) !=
                     hipSuccess) {
                 (void)hipGetLastError();
                 throw Error(SFFT_ERR_OUT_OF_MEMORY, "serving work-list buffers");
             }
-            check_hip(hipMemset(st.d_counters, 0, 2 * sizeof(unsigned)), "counter init");
+            check_hip(hipMemset(st.d_counters, 0, 2 * kPersistentClasses * sizeof(unsigned)),
+                      "counter init");
             check_hip(hipEventCreateWithFlags(&st.copy_done, hipEventDisableTiming), "event");
         }
         if (cfg_.plain) return;
@@ -268,7 +270,7 @@ class Server {
             (void)hipGetLastError();
             throw Error(SFFT_ERR_OUT_OF_MEMORY, "serving descriptor table");
         }
-        const size_t lds_cap = device_info().lds_bytes - kLdsReserve;
+        const size_t lds_cap = device_info().lds_bytes;
         if (cfg_.precision == kF64)
             table64_ = std::make_unique<UnitTable<double>>(lds_cap);
         else
@@ -402,7 +404,8 @@ class Server {
         }
         std::vector<WorkUnit> units;
         std::vector<int64_t> gin, gout;
-        b.finish(units, gin, gout);
+        size_t per_class[kPersistentClasses];
+        b.finish(units, gin, gout, per_class);
         if (units.size() > kMaxUnits || gin.size() > kMaxGather || rs.size() > kMaxFills)
             return false;
         sync_table(table);
@@ -429,12 +432,19 @@ class Server {
         check_hip(fill_synthetic_many(reinterpret_cast<const FillEntry *>(st.d_block + ub + 2 * gb),
                                       (int)rs.size(), st.s),
                   "fill");
-        const unsigned grid = persistent_grid<T>(table.lds_need(), units.size());
-        check_hip(launch_persistent<T>(du, (int)units.size(),
-                                       static_cast<const PassArgs<T> *>(d_table_), dgi, dgo,
-                                       st.d_counters, pool_->base(), pool_->base(), grid,
-                                       table.lds_need(), st.s),
-                  "merged launch");
+        // one launch per kernel class, in class order
+        size_t first = 0;
+        for (int c = 0; c < kPersistentClasses; ++c) {
+            if (!per_class[c]) continue;
+            const size_t lds = table.lds_need(c);
+            const unsigned grid = persistent_grid<T>(c, lds, per_class[c]);
+            check_hip(launch_persistent<T>(c, du + first, (int)per_class[c],
+                                           static_cast<const PassArgs<T> *>(d_table_), dgi, dgo,
+                                           st.d_counters + 2 * c, pool_->base(), pool_->base(),
+                                           grid, lds, st.s),
+                      "merged launch");
+            first += per_class[c];
+        }
         for (Req *r : rs) {
             r->ev = get_event();
             check_hip(hipEventRecord(r->ev, st.s), "event record");
