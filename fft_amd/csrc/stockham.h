@@ -176,6 +176,13 @@ struct Block {
     __device__ __forceinline__ int64_t obase(int g) const {
         return gather_out ? gather_out[g] : gbase(iout_, g);
     }
+    // byte address of element `elem` of an HBM buffer
+    __device__ __forceinline__ static const char *slot_ptr(const void *p, int64_t elem) {
+        return static_cast<const char *>(p) + elem * St::kBytes;
+    }
+    __device__ __forceinline__ static char *slot_ptr(void *p, int64_t elem) {
+        return static_cast<char *>(p) + elem * St::kBytes;
+    }
 
     // transform base and in-transform offset of slot g's element 0
     __device__ __forceinline__ void split_slot(const IoDesc &io, int g, int64_t &tbase,
@@ -267,15 +274,18 @@ struct Block {
                 }
             }
         } else if (src_global) {
-            const int64_t base = active ? ibase(g) : 0;
-            const int64_t es = iin_.elem_stride;
+            // the slot's first element, then 32-bit element offsets within the
+            // transform (a transform spans fewer than 2^32 elements): one
+            // 32-bit add and one 64-bit shift-add per element address
+            const char *sp = slot_ptr(in, active ? ibase(g) : 0);
+            const uint32_t es = (uint32_t)iin_.elem_stride, rstep = (uint32_t)nb * es;
 #pragma unroll
             for (int b = 0; b < B; ++b) {
-                const int j = lid + b * tpt_;
+                const uint32_t j = (uint32_t)(lid + b * tpt_);
                 if (ok(b)) {
 #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        v[b][r] = St::load(in, base + (int64_t)(j + r * nb) * es);
+                        v[b][r] = St::load(sp, (int64_t)(j * es + (uint32_t)r * rstep));
                 }
             }
             if (conj_in_) {
@@ -289,13 +299,13 @@ struct Block {
                 // pass), or by its index in the whole transform (first of two)
                 int64_t tb = 0, cofs = 0;
                 if (active && mode_ == kModeR2CLoad) split_slot(iin_, g, tb, cofs);
-                const int64_t rs = mode_ == kModeR2CLoad ? es : 1;
+                const uint32_t rs = mode_ == kModeR2CLoad ? es : 1u;
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
-                    const int j = ok(b) ? lid + b * tpt_ : 0;
+                    const uint32_t j = ok(b) ? (uint32_t)(lid + b * tpt_) : 0u;
 #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        v[b][r] = cmul(v[b][r], rot_[cofs + (int64_t)(j + r * nb) * rs]);
+                        v[b][r] = cmul(v[b][r], rot_[(uint32_t)cofs + (j + (uint32_t)(r * nb)) * rs]);
                 }
             }
         } else {
@@ -311,22 +321,31 @@ struct Block {
         }
         // ---- twiddle + butterfly
         if (ns > 1) {
+            // all twiddle loads of the stage, then (scheduling barrier: the
+            // compiler would otherwise pair each load with its use) the
+            // products
+            C2<T> w[B][R > 1 ? R - 1 : 1];
 #pragma unroll
             for (int b = 0; b < B; ++b) {
                 const int j = ok(b) ? lid + b * tpt_ : 0;
                 const uint32_t q = fdiv((uint32_t)j, st.div);
-                const int k = j - (int)q * ns;
-                const C2<T> *wp = tw_ + st.tw_off + k * tw_group(R);
+                const uint32_t k = (uint32_t)j - q * (uint32_t)ns;
+                const C2<T> *wp = tw_ + (uint32_t)(st.tw_off + k * (uint32_t)tw_group(R));
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], wp[r - 1]);
+                for (int r = 1; r < R; ++r) w[b][r - 1] = wp[r - 1];
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int b = 0; b < B; ++b)
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[b][r - 1]);
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) dft<R, T>(v[b]);
         // ---- store
         if (dst_global) {
-            const int64_t base = active ? obase(g) : 0;
-            const int64_t es = iout_.elem_stride;
+            char *sp = slot_ptr(out, active ? obase(g) : 0);
+            const uint32_t es = (uint32_t)iout_.elem_stride, rstep = (uint32_t)ns * es;
             if (conj_out_) {
 #pragma unroll
                 for (int b = 0; b < B; ++b)
@@ -336,7 +355,7 @@ struct Block {
             if (half_shift_ && (mode_ == kModeC2R || mode_ == kModeC2RStore)) {
                 int64_t tbo = 0, cofo = 0;
                 if (active && mode_ == kModeC2RStore) split_slot(iout_, g, tbo, cofo);
-                const int64_t rs = mode_ == kModeC2RStore ? es : 1;
+                const uint32_t rs = mode_ == kModeC2RStore ? es : 1u;
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
                     const int j = ok(b) ? lid + b * tpt_ : 0;
@@ -344,7 +363,7 @@ struct Block {
                     const int o = (int)q * ns * R + (j - (int)q * ns);
 #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        v[b][r] = cmulc(v[b][r], rot_[cofo + (int64_t)(o + r * ns) * rs]);
+                        v[b][r] = cmulc(v[b][r], rot_[(uint32_t)cofo + (uint32_t)(o + r * ns) * rs]);
                 }
             }
             if (post_tw_) {
@@ -353,17 +372,28 @@ struct Block {
                 const uint32_t c =
                     active ? (uint32_t)g - fdiv((uint32_t)g, iout_.per_batch) * iout_.per_batch.d : 0u;
                 const uint32_t mask = (1u << post_tw_shift_) - 1u;
+                // loads in groups of up to 8 elements, each group issued
+                // before its products (scheduling barriers as above)
+                constexpr int G = R < 8 ? R : 8;
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
                     const int j = ok(b) ? lid + b * tpt_ : 0;
                     const uint32_t q = fdiv((uint32_t)j, st.div);
                     const int o = (int)q * ns * R + (j - (int)q * ns);
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        // c < N2 and e < N1, so c e < N: no reduction needed
-                        const uint32_t m = c * (uint32_t)(o + r * ns);
-                        v[b][r] = cmul(v[b][r],
-                                       cmul(ptw_fine_[m & mask], ptw_coarse_[m >> post_tw_shift_]));
+                    for (int r0 = 0; r0 < R; r0 += G) {
+                        C2<T> fi[G], co[G];
+#pragma unroll
+                        for (int i = 0; i < G && r0 + i < R; ++i) {
+                            // c < N2 and e < N1, so c e < N: no reduction needed
+                            const uint32_t m = c * (uint32_t)(o + (r0 + i) * ns);
+                            fi[i] = ptw_fine_[m & mask];
+                            co[i] = ptw_coarse_[m >> post_tw_shift_];
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int i = 0; i < G && r0 + i < R; ++i)
+                            v[b][r0 + i] = cmul(v[b][r0 + i], cmul(fi[i], co[i]));
                     }
                 }
             }
@@ -372,10 +402,10 @@ struct Block {
                 const int j = lid + b * tpt_;
                 if (ok(b)) {
                     const uint32_t q = fdiv((uint32_t)j, st.div);
-                    const int o = (int)q * ns * R + (j - (int)q * ns);
+                    const uint32_t o = q * (uint32_t)(ns * R) + ((uint32_t)j - q * (uint32_t)ns);
 #pragma unroll
                     for (int r = 0; r < R; ++r)
-                        St::store(out, base + (int64_t)(o + r * ns) * es, v[b][r]);
+                        St::store(sp, (int64_t)(o * es + (uint32_t)r * rstep), v[b][r]);
                 }
             }
         } else {

@@ -57,6 +57,7 @@ struct Storage;
 template <typename T>
 struct Storage<T, StoreNative> {
     static constexpr bool native = true;
+    static constexpr int kBytes = (int)sizeof(C2<T>);  // bytes per complex value in HBM
     __device__ static C2<T> load(const void *p, int64_t i) {
         return static_cast<const C2<T> *>(p)[i];
     }
@@ -66,6 +67,7 @@ struct Storage<T, StoreNative> {
 template <>
 struct Storage<float, StoreF16> {
     static constexpr bool native = false;
+    static constexpr int kBytes = 4;
     __device__ static C2<float> load(const void *p, int64_t i) {
         const uint32_t w = static_cast<const uint32_t *>(p)[i];
         return {f16_to_f32(w & 0xffffu), f16_to_f32(w >> 16)};
@@ -78,6 +80,7 @@ struct Storage<float, StoreF16> {
 template <>
 struct Storage<float, StoreBF16> {
     static constexpr bool native = false;
+    static constexpr int kBytes = 4;
     __device__ static C2<float> load(const void *p, int64_t i) {
         const uint32_t w = static_cast<const uint32_t *>(p)[i];
         return {__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
@@ -94,6 +97,7 @@ struct Storage<float, StoreBF16> {
 template <>
 struct Storage<double, StoreF32> {
     static constexpr bool native = false;
+    static constexpr int kBytes = 8;
     __device__ static C2<double> load(const void *p, int64_t i) {
         const C2<float> v = static_cast<const C2<float> *>(p)[i];
         return {(double)v.x, (double)v.y};
