@@ -401,7 +401,7 @@ def precision_sweep(torch, fft_amd, reps=5):
     return out
 
 
-def serving(rates=(2000, 10000, 50000), duration=0.25):
+def serving(rates=(2000, 5000, 10000, 20000, 50000), duration=0.25):
     """Poisson arrivals: C2C fwd / inv / RealFFT fwd (1/3 each), fast sizes
     log-uniform in [256, 2^18], batch U{1..512}, per-request seeds; 64 GB
     HBM budget.  p50/p99 latency (arrival -> completion) and throughput, and

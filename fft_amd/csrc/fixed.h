@@ -415,17 +415,33 @@ struct Fixed {
                 const C2<T> v = x[0];
                 ls[0] = {v.x + v.y, -(v.x - v.y)};
             }
-            for (int i = (mod ? 0 : 1) + lid; i <= H / 2; i += TPT) {
-                if (real_pair_skipped(i, H, mod)) continue;
-                const int ci = mod ? (H - 1 - i) : (H - i);
-                const C2<T> v = x[i], v2 = x[ci];
-                const C2<T> odd = {v.x + v2.x, v.y - v2.y};
-                const C2<T> erot = {v.x - v2.x, v.y + v2.y};
-                const C2<T> ev = cmulc(erot, a.real_tw[i]);
-                const C2<T> p = cadd(odd, ev);
-                const C2<T> m = csub(odd, ev);
-                ls[pad(i)] = cconj(p);
-                ls[pad(ci)] = m;
+            // kRealBatch pairs at a time, their loads issued before any use
+            for (int i0 = (mod ? 0 : 1) + lid; i0 <= H / 2; i0 += kRealBatch * TPT) {
+                C2<T> va[kRealBatch], vb[kRealBatch], tw[kRealBatch];
+#pragma unroll
+                for (int k = 0; k < kRealBatch; ++k) {
+                    const int i = i0 + k * TPT;
+                    if (i <= H / 2) {
+                        const int ci = mod ? (H - 1 - i) : (H - i);
+                        va[k] = x[i];
+                        vb[k] = x[ci];
+                        tw[k] = a.real_tw[i];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kRealBatch; ++k) {
+                    const int i = i0 + k * TPT;
+                    if (i > H / 2 || real_pair_skipped(i, H, mod)) continue;
+                    const int ci = mod ? (H - 1 - i) : (H - i);
+                    const C2<T> v = va[k], v2 = vb[k];
+                    const C2<T> odd = {v.x + v2.x, v.y - v2.y};
+                    const C2<T> erot = {v.x - v2.x, v.y + v2.y};
+                    const C2<T> ev = cmulc(erot, tw[k]);
+                    const C2<T> p = cadd(odd, ev);
+                    const C2<T> m = csub(odd, ev);
+                    ls[pad(i)] = cconj(p);
+                    ls[pad(ci)] = m;
+                }
             }
         }
         __syncthreads();
@@ -445,6 +461,8 @@ struct Fixed {
                 y[0] = {z.x + z.y, z.x - z.y};
             }
             const T half = (T)0.5;
+            // (one pair at a time: grouping the pairs' loads measured slower
+            // here, 4.15 -> 3.91 TB/s at 2048, unlike the pre-process)
             for (int i = (mod ? 0 : 1) + lid; i <= H / 2; i += TPT) {
                 if (real_pair_skipped(i, H, mod)) continue;
                 const int ci = mod ? (H - 1 - i) : (H - i);

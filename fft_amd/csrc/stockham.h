@@ -45,6 +45,11 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Pairs of the runtime real pre-/post-process whose loads are issued
+// together (the loop trip count is not a compile-time constant, so the
+// compiler would otherwise wait for each pair's loads before the next).
+constexpr int kRealBatch = 4;
+
 // Real-FFT fusion modes.
 enum : int32_t {
     kModeC2C = 0,
@@ -700,17 +705,33 @@ struct Block {
                 const C2<T> v = x[0];
                 ls[0] = {v.x + v.y, -(v.x - v.y)};
             }
-            for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += tpt_) {
-                if (real_pair_skipped(i, h, mod)) continue;
-                const int ci = mod ? (h - 1 - i) : (h - i);
-                const C2<T> v = x[i], v2 = x[ci];
-                const C2<T> odd = {v.x + v2.x, v.y - v2.y};
-                const C2<T> erot = {v.x - v2.x, v.y + v2.y};
-                const C2<T> ev = cmulc(erot, real_tw_[i]);
-                const C2<T> p = cadd(odd, ev);
-                const C2<T> m = csub(odd, ev);
-                ls[lds_idx(i)] = cconj(p);
-                ls[lds_idx(ci)] = m;  // conj(conj(odd - ev))
+            // kRealBatch pairs at a time, their loads issued before any use
+            for (int i0 = (mod ? 0 : 1) + lid; i0 <= h / 2; i0 += kRealBatch * tpt_) {
+                C2<T> va[kRealBatch], vb[kRealBatch], tw[kRealBatch];
+#pragma unroll
+                for (int k = 0; k < kRealBatch; ++k) {
+                    const int i = i0 + k * tpt_;
+                    if (i <= h / 2) {
+                        const int ci = mod ? (h - 1 - i) : (h - i);
+                        va[k] = x[i];
+                        vb[k] = x[ci];
+                        tw[k] = real_tw_[i];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kRealBatch; ++k) {
+                    const int i = i0 + k * tpt_;
+                    if (i > h / 2 || real_pair_skipped(i, h, mod)) continue;
+                    const int ci = mod ? (h - 1 - i) : (h - i);
+                    const C2<T> v = va[k], v2 = vb[k];
+                    const C2<T> odd = {v.x + v2.x, v.y - v2.y};
+                    const C2<T> erot = {v.x - v2.x, v.y + v2.y};
+                    const C2<T> ev = cmulc(erot, tw[k]);
+                    const C2<T> p = cadd(odd, ev);
+                    const C2<T> m = csub(odd, ev);
+                    ls[lds_idx(i)] = cconj(p);
+                    ls[lds_idx(ci)] = m;  // conj(conj(odd - ev))
+                }
             }
         }
         __syncthreads();
@@ -732,15 +753,30 @@ struct Block {
             y[0] = {z.x + z.y, z.x - z.y};
         }
         const T half = (T)0.5;
-        for (int i = (mod ? 0 : 1) + lid; i <= h / 2; i += tpt_) {
-            if (real_pair_skipped(i, h, mod)) continue;
-            const int ci = mod ? (h - 1 - i) : (h - i);
-            const C2<T> zi = ls[lds_idx(i)], zc = ls[lds_idx(ci)];
-            const C2<T> odd = {(zi.x + zc.x) * half, (zi.y - zc.y) * half};
-            const C2<T> ev = {(zi.x - zc.x) * half, (zi.y + zc.y) * half};
-            const C2<T> e = cmul(ev, real_tw_[i]);
-            y[i] = cadd(odd, e);
-            y[ci] = cconj(csub(odd, e));
+        for (int i0 = (mod ? 0 : 1) + lid; i0 <= h / 2; i0 += kRealBatch * tpt_) {
+            C2<T> za[kRealBatch], zb[kRealBatch], tw[kRealBatch];
+#pragma unroll
+            for (int k = 0; k < kRealBatch; ++k) {
+                const int i = i0 + k * tpt_;
+                if (i <= h / 2) {
+                    const int ci = mod ? (h - 1 - i) : (h - i);
+                    za[k] = ls[lds_idx(i)];
+                    zb[k] = ls[lds_idx(ci)];
+                    tw[k] = real_tw_[i];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kRealBatch; ++k) {
+                const int i = i0 + k * tpt_;
+                if (i > h / 2 || real_pair_skipped(i, h, mod)) continue;
+                const int ci = mod ? (h - 1 - i) : (h - i);
+                const C2<T> zi = za[k], zc = zb[k];
+                const C2<T> odd = {(zi.x + zc.x) * half, (zi.y - zc.y) * half};
+                const C2<T> ev = {(zi.x - zc.x) * half, (zi.y + zc.y) * half};
+                const C2<T> e = cmul(ev, tw[k]);
+                y[i] = cadd(odd, e);
+                y[ci] = cconj(csub(odd, e));
+            }
         }
     }
 
