@@ -3,16 +3,20 @@
 // block, Sched<...>, variant, radices...>; a two-pass plan whose column
 // length has one uses it for that pass (same radices as the runtime-planned
 // pass, so the same bits).  Column length 4096 keeps the runtime-planned
-// pass: its tiles of 2-4 columns measured faster there.
+// pass: a compiled 4096 column kernel (tiles of 4 columns) measured slower
+// (2^24: 1070 -> 921 GB/s).  Tile widths: 2048-length columns 8 wide and
+// 256-length columns 32 wide measured faster than half that (2^22 +15 %,
+// 2^21 +6 %, 2^15..2^16 +2-4 %); 512 and 1024 wider measured slower or
+// the same.
 #include "fixed.h"
 
 namespace sfft {
 namespace {
 const FixedCol<float, 128, 8, 32, Sched<0, 0, 0, 0>, 0, 16, 8> col128;
-const FixedCol<float, 256, 16, 16, Sched<0, 0, 0, 0>, 0, 16, 16> col256;
+const FixedCol<float, 256, 16, 32, Sched<0, 0, 0, 0>, 0, 16, 16> col256;
 const FixedCol<float, 512, 32, 16, Sched<0, 0, 0, 0>, 0, 16, 16, 2> col512;
 const FixedCol<float, 1024, 64, 8, Sched<0, 0, 0, 0>, 0, 16, 16, 4> col1024;
-const FixedCol<float, 2048, 128, 4, Sched<0, 0, 0, 0>, 0, 16, 16, 8> col2048;
+const FixedCol<float, 2048, 128, 8, Sched<0, 0, 0, 0>, 0, 16, 16, 8> col2048;
 // complex FP64 (16-byte values: half the columns per LDS budget)
 const FixedCol<double, 256, 16, 8, Sched<0, 0, 0, 0>, 0, 16, 16> col256d;
 const FixedCol<double, 512, 32, 8, Sched<0, 0, 0, 0>, 0, 16, 16, 2> col512d;
