@@ -4,8 +4,10 @@
 
 namespace sfft {
 
+// The float-butterfly instances keep 4 waves per SIMD (at most 128
+// registers); the MFMA instance needs more for its split operands.
 template <int EPT, typename S, bool MFMA>
-__global__ void __launch_bounds__(kMaxBlock16) k_pass16(PassArgs<float> a) {
+__global__ void __launch_bounds__(kMaxBlock16, MFMA ? 1 : 4) k_pass16(PassArgs<float> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<float> *lds = reinterpret_cast<C2<float> *>(smem_raw);
     Block<float, EPT, S, MFMA> blk(a, a.in, a.out, a.total_slots, lds, (int)threadIdx.x,
