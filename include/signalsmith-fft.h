@@ -133,6 +133,10 @@ class Engine {
                 next.chunks_ = bytes >= kChunkFrom ? kChunks : 1;
                 for (int k = 0; k < next.chunks_; ++k) check(sfft_event_create(&next.ev_[k]));
             }
+            // chirp-z lengths: the workspace of a host-staged call, allocated
+            // once here so that fft/ifft allocate nothing
+            check(sfft_plan_workspace_size(next.plan_, 1, 0, &next.ws_bytes_));
+            if (next.ws_bytes_) check(sfft_malloc_device(&next.ws_, next.ws_bytes_));
             check(sfft_stream_create(&next.stream_));
         }
         swap(next);
@@ -148,7 +152,8 @@ class Engine {
                     size_t out_bytes, CopyIn &&copy_in, CopyOut &&copy_out) {
         if (direct_) {
             copy_in(0, in_elems);
-            check(sfft_execute(plan_, hin_dev_, hout_dev_, 1, direction, stream_));
+            check(sfft_execute_workspace(plan_, hin_dev_, hout_dev_, 1, direction, ws_, ws_bytes_,
+                                         stream_));
             check(sfft_stream_synchronize(stream_));
             copy_out(0, out_elems);
             return;
@@ -162,7 +167,7 @@ class Engine {
             check(sfft_memcpy(di + e0 * in_bytes, hi + e0 * in_bytes, (e1 - e0) * in_bytes,
                               SFFT_COPY_H2D, stream_));
         }
-        check(sfft_execute(plan_, din_, dout_, 1, direction, stream_));
+        check(sfft_execute_workspace(plan_, din_, dout_, 1, direction, ws_, ws_bytes_, stream_));
         for (int k = 0; k < chunks_; ++k) {
             const size_t e0 = out_elems * k / chunks_, e1 = out_elems * (k + 1) / chunks_;
             if (e1 > e0)
@@ -191,6 +196,8 @@ class Engine {
         std::swap(hout_dev_, o.hout_dev_);
         std::swap(stream_, o.stream_);
         for (int k = 0; k < kChunks; ++k) std::swap(ev_[k], o.ev_[k]);
+        std::swap(ws_, o.ws_);
+        std::swap(ws_bytes_, o.ws_bytes_);
         std::swap(bytes_, o.bytes_);
         std::swap(direct_, o.direct_);
         std::swap(chunks_, o.chunks_);
@@ -200,19 +207,23 @@ class Engine {
         if (plan_) sfft_plan_destroy(plan_);
         if (din_) sfft_free_device(din_);
         if (dout_) sfft_free_device(dout_);
+        if (ws_) sfft_free_device(ws_);
         if (hin_) sfft_free_host(hin_);
         if (hout_) sfft_free_host(hout_);
         for (int k = 0; k < kChunks; ++k)
             if (ev_[k]) sfft_event_destroy(ev_[k]);
         if (stream_) sfft_stream_destroy(stream_);
         plan_ = nullptr;
-        din_ = dout_ = hin_ = hout_ = hin_dev_ = hout_dev_ = stream_ = nullptr;
+        din_ = dout_ = hin_ = hout_ = hin_dev_ = hout_dev_ = stream_ = ws_ = nullptr;
+        ws_bytes_ = 0;
         for (int k = 0; k < kChunks; ++k) ev_[k] = nullptr;
     }
     sfft_plan plan_ = nullptr;
     void *din_ = nullptr, *dout_ = nullptr, *hin_ = nullptr, *hout_ = nullptr;
     void *hin_dev_ = nullptr, *hout_dev_ = nullptr;
     void *stream_ = nullptr;
+    void *ws_ = nullptr;  // workspace of a host-staged call (chirp-z lengths)
+    size_t ws_bytes_ = 0;
     void *ev_[kChunks] = {};
     size_t bytes_ = 0;
     bool direct_ = false;
