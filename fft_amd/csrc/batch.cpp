@@ -78,7 +78,7 @@ int UnitTable<T>::entry(const Plan &p, int inv) {
         bool fits = true;
         for (size_t q = 0; q < p.passes().size() && fits; ++q) {
             PassArgs<T> a = p.pass_args<T>(q, inv);
-            size_t sl = p.slot_lds_bytes(q);
+            const size_t sl = p.slot_lds_bytes(q);
             if (p.passes()[q].ept > pc.ept || a.pd.tpt > pc.threads || sl > cap) {
                 fits = false;
                 break;
@@ -105,13 +105,6 @@ int UnitTable<T>::entry(const Plan &p, int inv) {
                 while (cols * 2 <= tpw) cols *= 2;
                 a.pd.tpw = std::max(a.pd.tpw, cols);
                 if (a.pd.tpw * a.pd.tpt > pc.threads || sl * (size_t)a.pd.tpw > cap) fits = false;
-            }
-            if (fits && sl) {
-                // LDS slot stride with the fewest bank conflicts for this
-                // block's lane layout (the results do not depend on it)
-                const size_t eb = sizeof(C2<T>);
-                a.pd.lds_stride = choose_lds_stride(a.pd, pc.ept, pc.threads, eb, cap);
-                sl = (size_t)a.pd.lds_stride * eb;
             }
             need = std::max(need, sl * (size_t)a.pd.tpw);
             ds.push_back(a);

@@ -168,75 +168,6 @@ static int floor_pow2(int x) {
     return p;
 }
 
-// LDS bank conflicts of a pass layout: 64 banks of 4 bytes, 8-byte accesses
-// served 32 lanes at a time.  For every LDS read and write instruction of
-// the first waves of a block, the largest number of lanes of a half-wave on
-// one bank, summed.  Mirrors the lane -> (slot, butterfly) mapping and the
-// addresses of Block::stage (slot stride `stride`, lds_idx padding inside a
-// slot).
-static long lds_conflicts(const PassDesc &pd, int ept, int block, int stride) {
-    const int last = pd.nstages - 1;
-    const int waves = std::min(4, (block + 63) / 64);
-    long cost = 0;
-    int banks[64];
-    auto lds_idx = [](int e) { return e + (e >> kLdsPadShift); };
-    for (int s = 0; s <= last; ++s) {
-        const int R = pd.st[s].radix, ns = pd.st[s].ns, nb = pd.n / R;
-        const int B = std::max(1, ept / R);
-        const bool col = (s == 0) ? pd.in_col != 0 : (s == last) ? pd.out_col != 0 : false;
-        for (int dir = 0; dir < 2; ++dir) {  // 0: this stage's reads, 1: its writes
-            if (dir == 0 && s == 0) continue;     // stage 0 reads HBM
-            if (dir == 1 && s == last) continue;  // the last stage writes HBM
-            for (int w = 0; w < waves; ++w)
-                for (int b = 0; b < B; ++b)
-                    for (int r = 0; r < R; ++r)
-                        for (int half = 0; half < 2; ++half) {
-                            std::fill(banks, banks + 64, 0);
-                            int worst = 0;
-                            for (int l = 0; l < 32; ++l) {
-                                const int t = w * 64 + half * 32 + l;
-                                if (t >= block) break;
-                                const int slot = col ? t % pd.tpw : t / pd.tpt;
-                                const int lid = col ? t / pd.tpw : t % pd.tpt;
-                                const int j = lid + b * pd.tpt;
-                                if (slot >= pd.tpw || lid >= pd.tpt || j >= nb) continue;
-                                int e;
-                                if (dir == 0) {
-                                    e = j + r * nb;
-                                } else {
-                                    const int q = j / ns;
-                                    e = q * ns * R + (j - q * ns) + r * ns;
-                                }
-                                const long a = 2L * ((long)slot * stride + lds_idx(e));
-                                for (int d = 0; d < 2; ++d) worst = std::max(worst, ++banks[(a + d) & 63]);
-                            }
-                            cost += worst;
-                        }
-        }
-    }
-    return cost;
-}
-
-// Slot stride (in elements, at least the padded length) with the fewest
-// simulated bank conflicts among the 32 strides that cover every bank
-// offset; the layout's LDS, tpw slots of it, must stay within cap bytes.
-// The stride does not change any result, only the LDS addresses.
-int choose_lds_stride(const PassDesc &pd, int ept, int block, size_t eb, size_t cap) {
-    const int base = lds_padded(pd.n);
-    if (pd.nstages < 2 || eb != 8) return base;
-    int best = base;
-    long best_cost = lds_conflicts(pd, ept, block, base);
-    for (int d = 1; d < 32 && best_cost > 0; ++d) {
-        if ((size_t)(base + d) * eb * (size_t)pd.tpw > cap) break;
-        const long c = lds_conflicts(pd, ept, block, base + d);
-        if (c < best_cost) {
-            best_cost = c;
-            best = base + d;
-        }
-    }
-    return best;
-}
-
 // Fills the stage part of a PassDesc for length n with the given radices.
 static void fill_stages(PassDesc &pd, int n, const std::vector<int> &radices) {
     pd.n = n;
@@ -505,19 +436,6 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     // footprint outweigh the matrix-core arithmetic in an HBM-bound pass)
     if (stored_16(prec) && h && h->mfma == 1) {
         for (auto &pp : passes_) enable_mfma(pp);
-    }
-    // LDS slot strides with the fewest bank conflicts (runtime-planned
-    // passes), within the LDS that keeps as many blocks per CU resident
-    for (auto &pp : passes_) {
-        if (pp.lds_bytes == 0) continue;
-        const size_t eb = elem_bytes(prec);
-        const size_t per_cu = std::max<size_t>(1, device_info().lds_bytes / pp.lds_bytes);
-        const size_t cap = std::min(lds_budget(), device_info().lds_bytes / per_cu);
-        const int st = choose_lds_stride(pp.pd, pp.ept, (int)pp.block, eb, cap);
-        if (st != pp.pd.lds_stride) {
-            pp.pd.lds_stride = st;
-            pp.lds_bytes = (size_t)st * eb * (size_t)pp.pd.tpw;
-        }
     }
     if (arith_f64(prec))
         build_tables<double>();

@@ -69,10 +69,6 @@ struct PassPlan {
     unsigned fixed_grid[8] = {};  // resident blocks per mode (kModeC2C ... kModeColCol)
 };
 
-// Slot stride of a pass's LDS layout with the fewest bank conflicts (same
-// results for any stride); tpw slots of eb-byte elements stay within cap.
-int choose_lds_stride(const PassDesc &pd, int ept, int block, size_t eb, size_t cap);
-
 // Overrides the planner's choice (autotuner candidates).
 struct PlanHints {
     std::vector<int> radices;  // first-pass radix order (empty: planner's)
