@@ -220,9 +220,11 @@ def fast_sizes_batched(torch, fft_amd, lo=64, hi=65536, target_bytes=256 << 20, 
     for t in bufs:
         fft_amd.fill(t, seed=11)
     rates, compiled = [], 0
-    for n in sizes:
+    # every plan first: plan creation (host work) does not leave the GPU idle
+    # between the timed launches
+    plans = [fft_amd.Plan(n, L.C2C, L.F32) for n in sizes]
+    for n, p in zip(sizes, plans):
         batch = max(1, cap // n)
-        p = fft_amd.Plan(n, L.C2C, L.F32)
         compiled += L.lib().sfft_plan_variants(n, L.C2C, L.F32) > 0
         for i in range(2):
             p.execute_ptr(bufs[i].data_ptr(), bufs[2 + i].data_ptr(), batch, 0, sh)
