@@ -84,13 +84,15 @@ hipError_t bs_post(const C2<T> *c, size_t m_len, C2<T> *out, int64_t out_dist, s
 // Counter-based splitmix64 inputs (same stream as fft_amd/synth.py).
 hipError_t fill_synthetic(void *dst, size_t count, int f64, int dist, uint64_t seed,
                           hipStream_t stream);
-// The same for n buffers in one launch; `entries` is device memory.
+// The same for n buffers in one launch; `entries` is device memory, and
+// max_count bounds their counts (it sizes the grid; f64 as for the entries).
 struct FillEntry {
     void *dst;
     uint64_t count;  // values of the real type
     uint64_t seed;
     int32_t f64, dist;
 };
-hipError_t fill_synthetic_many(const FillEntry *entries, int n, hipStream_t stream);
+hipError_t fill_synthetic_many(const FillEntry *entries, int n, uint64_t max_count, int f64,
+                               hipStream_t stream);
 
 }  // namespace sfft

@@ -202,21 +202,85 @@ __global__ void k_fill(T *dst, uint64_t count, int dist, uint64_t key) {
                (uint64_t)gridDim.x * blockDim.x);
 }
 
+// Uniform [-1, 1) value of word t = key + (i + 1) * golden, as fill_range
+// computes it, in 32-bit arithmetic: only the top 24 bits of the word are
+// used, and the final xor-shift of the finaliser leaves them unchanged
+// (z >> 31 is below 2^33), so they are the top bits of the second product,
+// whose high half needs no low product.  m * 2^-24 * 2 - 1 = (m - 2^23) * 2^-23
+// is exact in FP32 and FP64.
+template <typename T>
+__device__ __forceinline__ T uniform_pm1(uint64_t t) {
+    t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+    t ^= t >> 27;
+    const uint32_t lo = (uint32_t)t, hi = (uint32_t)(t >> 32);
+    const uint32_t top = __umulhi(lo, 0x133111EBu) + lo * 0x94D049BBu + hi * 0x133111EBu;
+    const int m = (int)(top >> 8) - (1 << 23);
+    return (T)m * (T)(1.0 / 8388608.0);
+}
+
+// Uniform fill with 16-byte stores: each thread writes V consecutive values
+// per step; the unaligned head (dst is T-aligned) and the tail are written
+// by the first threads one value at a time.
+template <typename T>
+__device__ __forceinline__ void fill_uniform_range(T *dst, uint64_t count, uint64_t key,
+                                                   uint64_t tid, uint64_t stride) {
+    constexpr int V = 16 / sizeof(T);
+    constexpr uint64_t G = 0x9E3779B97F4A7C15ull;
+    const uint64_t mis = ((uintptr_t)dst & 15) / sizeof(T);
+    const uint64_t want = mis ? (uint64_t)V - mis : 0, head = want < count ? want : count;
+    const uint64_t nv = (count - head) / V, tail = head + nv * V;
+    if (tid < head) dst[tid] = uniform_pm1<T>(key + (tid + 1) * G);
+    if (tid < count - tail) dst[tail + tid] = uniform_pm1<T>(key + (tail + tid + 1) * G);
+    using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    Vec *body = reinterpret_cast<Vec *>(dst + head);
+    uint64_t t = key + (head + tid * V + 1) * G;
+    const uint64_t step = stride * V * G;
+    for (uint64_t q = tid; q < nv; q += stride, t += step) {
+        T v[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) v[k] = uniform_pm1<T>(t + (uint64_t)k * G);
+        Vec o;
+        __builtin_memcpy(&o, v, sizeof(o));
+        body[q] = o;
+    }
+}
+
+template <typename T>
+__global__ void k_fill_uniform(T *dst, uint64_t count, uint64_t key) {
+    fill_uniform_range(dst, count, key, blockIdx.x * (uint64_t)blockDim.x + threadIdx.x,
+                       (uint64_t)gridDim.x * blockDim.x);
+}
+
 // One buffer per blockIdx.y.
 __global__ void k_fill_many(const FillEntry *entries) {
     const FillEntry e = entries[blockIdx.y];
     const uint64_t first = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t key = fill_key(e.seed);
-    if (e.f64)
+    if (e.dist == 0) {
+        if (e.f64)
+            fill_uniform_range(static_cast<double *>(e.dst), e.count, key, first, stride);
+        else
+            fill_uniform_range(static_cast<float *>(e.dst), e.count, key, first, stride);
+    } else if (e.f64) {
         fill_range(static_cast<double *>(e.dst), e.count, e.dist, key, first, stride);
-    else
+    } else {
         fill_range(static_cast<float *>(e.dst), e.count, e.dist, key, first, stride);
+    }
 }
 
-hipError_t fill_synthetic_many(const FillEntry *entries, int n, hipStream_t stream) {
+// workgroups of 256 threads for `count` values, 16 bytes per thread and step
+static unsigned fill_blocks(uint64_t count, int f64) {
+    const uint64_t per_block = 256ull * (f64 ? 2 : 4);
+    const uint64_t b = (count + per_block - 1) / per_block;
+    return (unsigned)(b < 1 ? 1 : b > 16384 ? 16384 : b);
+}
+
+hipError_t fill_synthetic_many(const FillEntry *entries, int n, uint64_t max_count, int f64,
+                               hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_many, dim3(256, (unsigned)n), dim3(256), 0, stream, entries);
+    hipLaunchKernelGGL(k_fill_many, dim3(fill_blocks(max_count, f64), (unsigned)n), dim3(256), 0,
+                       stream, entries);
     return hipGetLastError();
 }
 
@@ -224,6 +288,16 @@ hipError_t fill_synthetic(void *dst, size_t count, int f64, int dist, uint64_t s
                           hipStream_t stream) {
     if (count == 0) return hipSuccess;
     const uint64_t key = fill_key(seed);
+    if (dist == 0 && (uintptr_t)dst % (f64 ? 8 : 4) == 0) {
+        const unsigned blocks = fill_blocks(count, f64);
+        if (f64)
+            hipLaunchKernelGGL((k_fill_uniform<double>), dim3(blocks), dim3(256), 0, stream,
+                               (double *)dst, (uint64_t)count, key);
+        else
+            hipLaunchKernelGGL((k_fill_uniform<float>), dim3(blocks), dim3(256), 0, stream,
+                               (float *)dst, (uint64_t)count, key);
+        return hipGetLastError();
+    }
     size_t blocks = (count + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (f64)

@@ -419,9 +419,12 @@ class Server {
         FillEntry *fe = reinterpret_cast<FillEntry *>(h + ub + 2 * gb);
         const size_t fb = rs.size() * sizeof(FillEntry);
         const int f64 = cfg_.precision == kF64;
-        for (size_t i = 0; i < rs.size(); ++i)
+        uint64_t max_count = 0;
+        for (size_t i = 0; i < rs.size(); ++i) {
             fe[i] = FillEntry{pool_->base() + rs[i]->in_off,
                               (uint64_t)(rs[i]->in_bytes / (f64 ? 8 : 4)), rs[i]->r.seed, f64, 0};
+            max_count = std::max(max_count, fe[i].count);
+        }
         check_hip(hipMemcpyAsync(st.d_block, h, ub + 2 * gb + fb, hipMemcpyHostToDevice, st.s),
                   "work-list upload");
         check_hip(hipEventRecord(st.copy_done, st.s), "event record");
@@ -430,7 +433,7 @@ class Server {
         const int64_t *dgi = reinterpret_cast<const int64_t *>(st.d_block + ub);
         const int64_t *dgo = reinterpret_cast<const int64_t *>(st.d_block + ub + gb);
         check_hip(fill_synthetic_many(reinterpret_cast<const FillEntry *>(st.d_block + ub + 2 * gb),
-                                      (int)rs.size(), st.s),
+                                      (int)rs.size(), max_count, f64, st.s),
                   "fill");
         // one launch per kernel class, in class order
         size_t first = 0;
