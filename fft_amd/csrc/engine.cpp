@@ -154,6 +154,11 @@ static size_t bluestein_length(size_t n) {
 }
 
 static size_t elem_bytes(int prec) { return arith_f64(prec) ? 16 : 8; }
+// Two-pass plans up to this many bytes of elements keep a full table of
+// their four-step factors (one entry per element, held in L2 while a batch
+// streams through).  Larger plans use the two-level fine x coarse table,
+// whose gathers cost more per element but whose size does not grow with N.
+constexpr size_t kFullPostTwBytes = 2u << 20;
 static int max_block(int prec) {
     return stored_16(prec) ? kMaxBlock16 : (arith_f64(prec) ? kMaxBlock<double>() : kMaxBlock<float>());
 }
@@ -507,6 +512,7 @@ Plan::~Plan() {
     if (d_rot_) (void)hipFree(d_rot_);
     if (d_ptw_fine_) (void)hipFree(d_ptw_fine_);
     if (d_ptw_coarse_) (void)hipFree(d_ptw_coarse_);
+    if (d_ptw_full_) (void)hipFree(d_ptw_full_);
 }
 
 template <typename T>
@@ -551,7 +557,13 @@ void Plan::build_tables() {
                                        kind_ == kR2CHalf, s),
                   "real twiddle generation");
     }
-    if (passes_.size() > 1) {
+    if (passes_.size() > 1 && cn_ * sizeof(C2<T>) <= kFullPostTwBytes) {
+        // one entry per element: the first pass reads its factors with the
+        // same lane pattern as its stores, instead of two gathers each
+        alloc(&d_ptw_full_, cn_ * sizeof(C2<T>));
+        check_hip(gen_post_twiddles_full<T>((C2<T> *)d_ptw_full_, (size_t)passes_[0].pd.n, cn_, s),
+                  "post twiddle generation");
+    } else if (passes_.size() > 1) {
         int shift = 0;
         while ((1ull << (2 * shift)) < cn_) ++shift;
         ptw_shift_ = shift;
@@ -617,6 +629,7 @@ PassArgs<T> Plan::pass_args(size_t p, int inverse) const {
     a.rot = (const C2<T> *)d_rot_;
     a.ptw_fine = (const C2<T> *)d_ptw_fine_;
     a.ptw_coarse = (const C2<T> *)d_ptw_coarse_;
+    a.ptw_full = (const C2<T> *)d_ptw_full_;
     a.in_from_out = pp.in_from_out;
     return a;
 }
@@ -783,6 +796,7 @@ void Plan::run_fixed_columns(size_t p, const void *src, int64_t dsrc, void *dst,
         a.ptw_shift = ptw_shift_;
         a.ptw_fine = (const C2<T> *)d_ptw_fine_;
         a.ptw_coarse = (const C2<T> *)d_ptw_coarse_;
+        a.ptw_full = (const C2<T> *)d_ptw_full_;
         const size_t groups = ((size_t)a.total_slots + fk.tpw - 1) / fk.tpw;
         const unsigned grid = (unsigned)std::min<size_t>(groups, pp.fixed_grid[mode]);
         check_hip(fk.launch(&a, mode, grid, stream), "column-pass launch");

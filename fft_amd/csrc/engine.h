@@ -161,6 +161,7 @@ class Plan {
     void *d_rot_ = nullptr;
     void *d_ptw_fine_ = nullptr;  // C2<T>, the plan's arithmetic type
     void *d_ptw_coarse_ = nullptr;
+    void *d_ptw_full_ = nullptr;  // full four-step table, small plans only
     int ptw_shift_ = 0;
     size_t table_bytes_ = 0;
     std::unique_ptr<Plan> bs_inner_;

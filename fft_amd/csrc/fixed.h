@@ -38,6 +38,7 @@ struct FixedArgs {
     int32_t out_bs, out_ss, out_es;
     int32_t post_tw, ptw_shift;  // first pass: output element e times W_N^{c e}
     const C2<T> *ptw_fine, *ptw_coarse;
+    const C2<T> *ptw_full;  // or null: W_N^{c e} at c N + e (N: this pass' length)
 };
 
 // Sign flip of the imaginary part by a uniform xor mask (0 or the sign bit).
@@ -322,7 +323,20 @@ struct Fixed {
             for (int b = 0; b < B; ++b)
 #pragma unroll
                 for (int t2 = 0; t2 < R; ++t2) flip_im(v[b][t2], a.flip_out);
-            if (a.post_tw) {
+            if (a.post_tw && a.ptw_full) {
+                // the four-step factors from the full table: the lanes of a
+                // slot read consecutive entries (lanes without a butterfly
+                // read entry 0; never stored)
+                const uint32_t crow = active ? c * (uint32_t)N : 0u;
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = (active && jvalid(b)) ? lid + b * TPT : 0;
+                    const int o = (j / NS) * NS * R + j % NS;
+#pragma unroll
+                    for (int t2 = 0; t2 < R; ++t2)
+                        v[b][t2] = cmul(v[b][t2], a.ptw_full[crow + (uint32_t)(o + t2 * NS)]);
+                }
+            } else if (a.post_tw) {
                 // the four-step factors of every element, loaded together
                 // (lanes without a butterfly read entry 0; never stored)
 #pragma unroll
@@ -485,13 +499,19 @@ struct Fixed {
         const int ngroups = (a.total_slots + TPW - 1) / TPW;
         C2<T> pf[PF];
         if constexpr (COL) {
-            // Column tiles narrower than 128 bytes share HBM lines with their
-            // neighbours: then each window of gridDim.x groups is numbered so
-            // that every XCD (blocks go round-robin over them) runs a
+            // Column tiles share HBM lines with their neighbours when a tile
+            // is narrower than 128 bytes, or when the rows of a strided side
+            // do not start on 128-byte lines (a row pitch that is no multiple
+            // of 128 bytes): then each window of gridDim.x groups is numbered
+            // so that every XCD (blocks go round-robin over them) runs a
             // contiguous run of groups, whose shared lines meet in one L2.
             const int G = (int)gridDim.x;
             constexpr bool NARROW = TPW * (int)DB < 128;
-            const int off = NARROW ? (int)xcd_tile(blockIdx.x, (unsigned)G) : (int)blockIdx.x;
+            const auto odd_pitch = [](int32_t es) {
+                return es != 1 && (((uint32_t)es * DB) & 127u) != 0;
+            };
+            const bool share = NARROW || odd_pitch(a.in_es) || odd_pitch(a.out_es);
+            const int off = share ? (int)xcd_tile(blockIdx.x, (unsigned)G) : (int)blockIdx.x;
             int grp = off;
             if (grp < ngroups) load0(a, pf, grp * TPW + slot0, lid0);
             for (int w = 0; w < ngroups; w += G, grp += G) {

@@ -63,6 +63,10 @@ hipError_t gen_real_twiddles(C2<T> *tmi, C2<T> *rot, size_t n, int half_shift,
 template <typename T>
 hipError_t gen_post_twiddles(C2<T> *fine, C2<T> *coarse, int shift, size_t N,
                              hipStream_t stream);
+// Full table full[c n1 + e] = W_N^{c e} for c < N / n1, e < n1 (FP64, rounded
+// once to T).
+template <typename T>
+hipError_t gen_post_twiddles_full(C2<T> *full, size_t n1, size_t N, hipStream_t stream);
 
 // Bluestein (chirp-z) helpers, bluestein.hip.  w: chirp of length N;
 // b: convolution kernel of length M; a / c: per-transform rows of length M.

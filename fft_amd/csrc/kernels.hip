@@ -154,6 +154,25 @@ hipError_t gen_post_twiddles(C2<T> *fine, C2<T> *coarse, int shift, size_t N,
                        fine, coarse, shift, (uint64_t)N);
     return hipGetLastError();
 }
+template <typename T>
+__global__ void k_post_tw_full(C2<T> *full, uint32_t n1, uint64_t N) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < N) {
+        const uint64_t c = i / n1, e = i - c * n1;
+        const C2<double> w = unit_root(c * e, N);
+        full[i] = {(T)w.x, (T)w.y};
+    }
+}
+
+template <typename T>
+hipError_t gen_post_twiddles_full(C2<T> *full, size_t n1, size_t N, hipStream_t stream) {
+    hipLaunchKernelGGL((k_post_tw_full<T>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, stream,
+                       full, (uint32_t)n1, (uint64_t)N);
+    return hipGetLastError();
+}
+template hipError_t gen_post_twiddles_full<float>(C2<float> *, size_t, size_t, hipStream_t);
+template hipError_t gen_post_twiddles_full<double>(C2<double> *, size_t, size_t, hipStream_t);
+
 template hipError_t gen_post_twiddles<float>(C2<float> *, C2<float> *, int, size_t, hipStream_t);
 template hipError_t gen_post_twiddles<double>(C2<double> *, C2<double> *, int, size_t,
                                               hipStream_t);

@@ -91,6 +91,8 @@ struct PassArgs {
     const C2<T> *rot;      // half-shift pre-rotations e^{-2 pi i i / N}, N/2 entries
     const C2<T> *ptw_fine;    // W_N^{m}, m < 2^shift (FP64 values rounded once to T)
     const C2<T> *ptw_coarse;  // W_N^{m 2^shift}
+    const C2<T> *ptw_full;    // or null: W_N^{c e} at c n + e for every column c and
+                              // element e (FP64 values rounded once to T)
     int32_t in_from_out;   // second pass of a two-pass plan: reads the output buffer
     int32_t real_h;        // complex length h of a two-pass real transform
 };
@@ -146,7 +148,7 @@ struct Block {
     int conj_in_, conj_out_, mode_, half_shift_, post_tw_, post_tw_shift_, real_h_, mfma_;
     IoDesc iin_, iout_;
     const C2<T> *tw_, *real_tw_, *rot_;
-    const C2<T> *ptw_fine_, *ptw_coarse_;
+    const C2<T> *ptw_fine_, *ptw_coarse_, *ptw_full_;
 
     __device__ Block(const PassArgs<T> &args, const C2<T> *in_, C2<T> *out_, int total, C2<T> *l,
                      int t, int s0)
@@ -173,6 +175,7 @@ struct Block {
         rot_ = args.rot;
         ptw_fine_ = args.ptw_fine;
         ptw_coarse_ = args.ptw_coarse;
+        ptw_full_ = args.ptw_full;
     }
 
     __device__ __forceinline__ int64_t ibase(int g) const {
@@ -212,7 +215,9 @@ struct Block {
     __device__ __forceinline__ C2<T> post_twiddle(int g, int e) const {
         // W_N^{c e}, c = column index of slot g, as the product of two table
         // entries (fine x coarse), each an FP64 value rounded once to T.
+        // (or the entry of the full table)
         const uint32_t c = (uint32_t)g - fdiv((uint32_t)g, iout_.per_batch) * iout_.per_batch.d;
+        if (ptw_full_) return ptw_full_[c * (uint32_t)n_ + (uint32_t)e];
         // c < N2 and e < N1, so c e < N: no reduction needed.
         const uint32_t m = c * (uint32_t)e;
         return cmul(ptw_fine_[m & ((1u << post_tw_shift_) - 1u)], ptw_coarse_[m >> post_tw_shift_]);
@@ -371,7 +376,32 @@ struct Block {
                         v[b][r] = cmulc(v[b][r], rot_[(uint32_t)cofo + (uint32_t)(o + r * ns) * rs]);
                 }
             }
-            if (post_tw_) {
+            if (post_tw_ && ptw_full_) {
+                // four-step twiddle W_N^{c e} (c: column of slot g) from the
+                // full table: lanes of a slot read consecutive entries
+                const uint32_t crow =
+                    active ? ((uint32_t)g - fdiv((uint32_t)g, iout_.per_batch) * iout_.per_batch.d) *
+                                 (uint32_t)n_
+                           : 0u;
+                constexpr int G = R < 8 ? R : 8;
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const int j = ok(b) ? lid + b * tpt_ : 0;
+                    const uint32_t q = fdiv((uint32_t)j, st.div);
+                    const int o = (int)q * ns * R + (j - (int)q * ns);
+#pragma unroll
+                    for (int r0 = 0; r0 < R; r0 += G) {
+                        C2<T> fu[G];
+#pragma unroll
+                        for (int i = 0; i < G && r0 + i < R; ++i)
+                            fu[i] = ptw_full_[crow + (uint32_t)(o + (r0 + i) * ns)];
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int i = 0; i < G && r0 + i < R; ++i)
+                            v[b][r0 + i] = cmul(v[b][r0 + i], fu[i]);
+                    }
+                }
+            } else if (post_tw_) {
                 // four-step twiddle W_N^{c e} (c: column of slot g) as the
                 // product of a fine and a coarse table entry
                 const uint32_t c =
