@@ -161,6 +161,9 @@ static size_t elem_bytes(int prec) { return arith_f64(prec) ? 16 : 8; }
 // whose size does not grow with N (a full table at 2^24 F32 measured 1.43 ->
 // 1.19 TB/s: the single transform reads its 128 MB table once).
 constexpr size_t kFullPostTwBytes = 8u << 20;
+// Complex FP32 length from which a runtime-planned single pass gives way to
+// a two-pass split over compiled column kernels (Plan::Plan).
+constexpr size_t kTwoPassFrom = 16000;
 static int max_block(int prec) {
     return stored_16(prec) ? kMaxBlock16 : (arith_f64(prec) ? kMaxBlock<double>() : kMaxBlock<float>());
 }
@@ -374,14 +377,21 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     }
     if (!ok && !force_two && cn_ <= (1u << 20))
         ok = layout_single((int)cn_, prec, kind != kC2C, h, single);
-    if (ok) {
-        passes_.push_back(single);
-    } else {
-        if (force_one) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "does not fit one pass");
+    // A long runtime-planned single pass (no compiled kernel, one transform
+    // per CU) runs slower than a two-pass split over compiled column kernels
+    // (C2C F32, 256 MB batches: 16200 1.46 -> 2.23 TB/s, 16000 1.59 -> 2.10,
+    // 18432 1.70 -> 2.54; below 16000 the two were mixed, 15625 0.65x).  From
+    // kTwoPassFrom such lengths take a split whose two column lengths both
+    // have compiled kernels, where one exists; hinted plans keep the rule
+    // below (one pass where it fits).
+    const bool compiled_split = ok && !single.fixed && !h && kind == kC2C && prec == kF32 &&
+                                cn_ >= kTwoPassFrom;
+    size_t best_n1 = 0;
+    if (!ok || compiled_split) {
+        if (!ok && force_one) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "does not fit one pass");
         // Two passes: cn = N1 * N2.  Pass 1: N2 column FFTs of length N1
         // (stride N2) with W_N^{n2 k1}, written transposed as rows; pass 2:
         // N1 column FFTs of length N2 in place (stride N1).
-        size_t best_n1 = 0;
         double best_score = 1e30;
         // column tile width: a hint's transforms_per_block caps it.  8 columns
         // (64 B rows in F32) measured faster than 16 on most lengths: smaller
@@ -391,6 +401,8 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
             if (cn_ % n1) continue;
             if (h && h->split > 1 && n1 != h->split) continue;
             const size_t n2 = cn_ / n1;
+            if (compiled_split && (!find_fixed((int)n1, prec, 2, -1) || !find_fixed((int)n2, prec, 2, -1)))
+                continue;
             PassPlan p1, p2;
             if (!layout_columns((int)n1, prec, (int)std::min(n2, max_cols), h, p1)) continue;
             if (!layout_columns((int)n2, prec, (int)std::min(n1, max_cols), h, p2)) continue;
@@ -403,10 +415,15 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
                 best_n1 = n1;
             }
         }
-        if (!best_n1)
+        if (!best_n1 && !ok)
             throw Error(SFFT_ERR_UNSUPPORTED_SIZE,
                         "size " + std::to_string(n) + ": no two-pass split fits LDS (F64 plans "
                         "reach 2^26, F32 2^28)");
+    }
+    if (!best_n1) {
+        passes_.push_back(single);
+    } else {
+        const size_t max_cols = (h && h->tpw) ? (size_t)h->tpw : 8;
         const size_t n1 = best_n1, n2 = cn_ / n1;
         PassPlan p1, p2;
         layout_columns((int)n1, prec, (int)std::min(n2, max_cols), h, p1);
