@@ -170,11 +170,15 @@ constexpr size_t kTwoPassFrom = 16000;
 // have compiled kernels was timed (batched, 256 MB launches), and a length
 // is listed where its fastest split beat the planner's scored choice by
 // more than 5 % (81 of 185 lengths; up to 1.22x, median of all lengths 2.04
-// -> 2.17 TB/s).  Sorted by length.
+// -> 2.17 TB/s).  The first three lengths, below 16000, have no compiled
+// single pass and measured faster split than in one runtime-planned pass
+// (10935: 1.81 -> 2.14 TB/s, 13122: 1.61 -> 2.06, 15552: 1.79 -> 2.29).
+// Sorted by length.
 struct MeasuredSplit {
     uint32_t n, n1;
 };
 static const MeasuredSplit kMeasuredSplits[] = {
+    {10935, 135}, {13122, 81}, {15552, 192},
     {18225, 81}, {19683, 243}, {24000, 160}, {34560, 192}, {34992, 144}, {36000, 160},
     {38880, 160}, {39366, 243}, {43200, 192}, {46656, 192}, {46875, 375}, {51200, 256},
     {54675, 243}, {56250, 250}, {60000, 375}, {64000, 160}, {64800, 160}, {67500, 300},
@@ -446,9 +450,10 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
     // 18432 1.70 -> 2.54; below 16000 the two were mixed, 15625 0.65x).  From
     // kTwoPassFrom such lengths take a split whose two column lengths both
     // have compiled kernels, where one exists; hinted plans keep the rule
-    // below (one pass where it fits).
+    // below (one pass where it fits).  Shorter lengths listed in the
+    // measured split table (below) measured faster as two passes too.
     const bool compiled_split = ok && !single.fixed && !h && kind == kC2C && prec == kF32 &&
-                                cn_ >= kTwoPassFrom;
+                                (cn_ >= kTwoPassFrom || lookup_split(kMeasuredSplits, cn_));
     size_t best_n1 = 0;
     if (!ok || compiled_split) {
         if (!ok && force_one) throw Error(SFFT_ERR_UNSUPPORTED_SIZE, "does not fit one pass");

@@ -1,5 +1,5 @@
 // fixed_single0.hip -- compile-time plans, complex FP32, single pass: part
-// 1 of 4 of the fast lengths 2^a 3^b 5^c in [270, 19200] that have no
+// 1 of 4 of the fast lengths 2^a 3^b 5^c in [64, 19200] that have no
 // hand-tuned kernel.  For each length two candidates were measured against the
 // runtime-planned pass (batched launches of about 256 MB, cold inputs): the
 // radices the runtime cost model picks at 16 elements per thread and at 32
@@ -7,8 +7,8 @@
 // about 256 threads per block.  A length is listed only where its faster
 // candidate beat the runtime pass by 5 % or more (the rate pair in GB/s, runtime
 // -> compiled, follows each line).  Lengths where the candidates lost (up to
-// 2.4x slower, mostly lengths above 10000 at 600 to 1000 threads per
-// transform) keep the runtime pass.
+// 2.4x slower: spilled registers at up to 1024 threads per transform) keep the
+// runtime pass.
 // FixedC2C<type, N, threads per transform, transforms per block,
 //          Sched<prefetch stage, min waves per SIMD, pair loads, prefetch first,
 //          LDS pad shift>, variant, radices...>
@@ -16,28 +16,30 @@
 
 namespace sfft {
 namespace {
-const FixedC2C<float, 270, 23, 11, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 3> c270;  // 4145 -> 4464
-const FixedC2C<float, 384, 16, 16, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 12> c384;  // 4657 -> 4974
-const FixedC2C<float, 540, 45, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 6> c540;  // 4116 -> 4361
-const FixedC2C<float, 800, 32, 8, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25> c800;  // 4328 -> 4811
-const FixedC2C<float, 1000, 100, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 10> c1000;  // 4081 -> 4604
-const FixedC2C<float, 1440, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 10> c1440;  // 3897 -> 4952
-const FixedC2C<float, 1600, 54, 4, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 10, 5> c1600;  // 4194 -> 4438
-const FixedC2C<float, 2025, 81, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 9, 9> c2025;  // 3247 -> 4518
-const FixedC2C<float, 2400, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 10> c2400;  // 4077 -> 4806
-const FixedC2C<float, 2700, 225, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 12> c2700;  // 4227 -> 4513
-const FixedC2C<float, 3125, 125, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 25, 5> c3125;  // 4028 -> 4646
-const FixedC2C<float, 3888, 432, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 6, 6> c3888;  // 2701 -> 4333
-const FixedC2C<float, 4374, 486, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 6> c4374;  // 2952 -> 4376
-const FixedC2C<float, 4860, 540, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9, 6> c4860;  // 2075 -> 3879
-const FixedC2C<float, 5625, 375, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 15, 15> c5625;  // 3456 -> 3886
-const FixedC2C<float, 6075, 405, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 3, 3, 3> c6075;  // 2612 -> 3175
-const FixedC2C<float, 6480, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 9, 9, 5> c6480;  // 2106 -> 4432
-const FixedC2C<float, 7290, 810, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9, 9> c7290;  // 2494 -> 4064
-const FixedC2C<float, 8100, 900, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 9, 9> c8100;  // 2428 -> 3909
-const FixedC2C<float, 9600, 640, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 8, 5> c9600;  // 2347 -> 3779
-const FixedC2C<float, 10368, 864, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 12, 6> c10368;  // 2554 -> 3779
-const FixedC2C<float, 12000, 480, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 15> c12000;  // 2418 -> 3702
-const FixedC2C<float, 14400, 960, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 15, 4> c14400;  // 2646 -> 3489
+const FixedC2C<float, 64, 4, 64, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 4> c64;  // 4834 -> 5131
+const FixedC2C<float, 250, 10, 25, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 10> c250;  // 4122 -> 4544
+const FixedC2C<float, 324, 36, 7, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 6, 6> c324;  // 4004 -> 4571
+const FixedC2C<float, 500, 50, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 5> c500;  // 4545 -> 4950
+const FixedC2C<float, 729, 81, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9> c729;  // 4486 -> 4818
+const FixedC2C<float, 972, 108, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 9> c972;  // 4137 -> 4861
+const FixedC2C<float, 1296, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 9> c1296;  // 3627 -> 5000
+const FixedC2C<float, 1536, 128, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 6> c1536;  // 4514 -> 4749
+const FixedC2C<float, 1944, 216, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 6, 6, 6> c1944;  // 3116 -> 4305
+const FixedC2C<float, 2250, 225, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 10> c2250;  // 3852 -> 4323
+const FixedC2C<float, 2592, 96, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 9, 9> c2592;  // 3649 -> 4711
+const FixedC2C<float, 3072, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 12> c3072;  // 4616 -> 4856
+const FixedC2C<float, 3645, 405, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 5> c3645;  // 2636 -> 4313
+const FixedC2C<float, 4320, 160, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 9> c4320;  // 2651 -> 4137
+const FixedC2C<float, 4800, 160, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 10> c4800;  // 2699 -> 4347
+const FixedC2C<float, 5400, 360, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 8, 3> c5400;  // 2795 -> 3300
+const FixedC2C<float, 6000, 400, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 16, 15> c6000;  // 3631 -> 4166
+const FixedC2C<float, 6400, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 8> c6400;  // 3649 -> 4520
+const FixedC2C<float, 7200, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 15> c7200;  // 3394 -> 4331
+const FixedC2C<float, 8000, 320, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 10> c8000;  // 2433 -> 3788
+const FixedC2C<float, 9216, 768, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 6, 6> c9216;  // 2393 -> 3904
+const FixedC2C<float, 10240, 683, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 8, 5> c10240;  // 2426 -> 3817
+const FixedC2C<float, 11664, 432, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 9, 9, 9> c11664;  // 1962 -> 3545
+const FixedC2C<float, 12960, 480, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 9, 9, 5> c12960;  // 1917 -> 3446
+const FixedC2C<float, 19200, 640, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 10, 10, 6> c19200;  // 1818 -> 2153
 }  // namespace
 }  // namespace sfft

@@ -1,5 +1,5 @@
 // fixed_single1.hip -- compile-time plans, complex FP32, single pass: part
-// 2 of 4 of the fast lengths 2^a 3^b 5^c in [270, 19200] that have no
+// 2 of 4 of the fast lengths 2^a 3^b 5^c in [64, 19200] that have no
 // hand-tuned kernel.  For each length two candidates were measured against the
 // runtime-planned pass (batched launches of about 256 MB, cold inputs): the
 // radices the runtime cost model picks at 16 elements per thread and at 32
@@ -7,8 +7,8 @@
 // about 256 threads per block.  A length is listed only where its faster
 // candidate beat the runtime pass by 5 % or more (the rate pair in GB/s, runtime
 // -> compiled, follows each line).  Lengths where the candidates lost (up to
-// 2.4x slower, mostly lengths above 10000 at 600 to 1000 threads per
-// transform) keep the runtime pass.
+// 2.4x slower: spilled registers at up to 1024 threads per transform) keep the
+// runtime pass.
 // FixedC2C<type, N, threads per transform, transforms per block,
 //          Sched<prefetch stage, min waves per SIMD, pair loads, prefetch first,
 //          LDS pad shift>, variant, radices...>
@@ -16,28 +16,29 @@
 
 namespace sfft {
 namespace {
-const FixedC2C<float, 288, 24, 10, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 2> c288;  // 4478 -> 4880
-const FixedC2C<float, 405, 45, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 5> c405;  // 4007 -> 4673
-const FixedC2C<float, 576, 48, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 6, 6> c576;  // 4426 -> 4888
-const FixedC2C<float, 810, 90, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9> c810;  // 4286 -> 4849
-const FixedC2C<float, 1080, 90, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 12, 6> c1080;  // 4323 -> 4626
-const FixedC2C<float, 1458, 162, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 2> c1458;  // 3096 -> 4339
-const FixedC2C<float, 1620, 180, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 12, 9> c1620;  // 3928 -> 4622
-const FixedC2C<float, 2160, 180, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 12, 12> c2160;  // 4366 -> 4645
-const FixedC2C<float, 2430, 270, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9, 3> c2430;  // 2060 -> 3701
-const FixedC2C<float, 2880, 96, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 6> c2880;  // 4489 -> 4919
-const FixedC2C<float, 3240, 270, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 6, 6> c3240;  // 2167 -> 3050
-const FixedC2C<float, 4000, 160, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 5> c4000;  // 2794 -> 4690
-const FixedC2C<float, 4500, 188, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 15, 12> c4500;  // 2461 -> 3238
-const FixedC2C<float, 5120, 171, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 5> c5120;  // 2876 -> 4420
-const FixedC2C<float, 5760, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 12> c5760;  // 3084 -> 4608
-const FixedC2C<float, 6144, 205, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 6> c6144;  // 3192 -> 4298
-const FixedC2C<float, 6561, 729, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 9> c6561;  // 2575 -> 4060
-const FixedC2C<float, 7680, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 16, 15> c7680;  // 3422 -> 4318
-const FixedC2C<float, 8640, 720, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 12, 5> c8640;  // 2257 -> 3813
-const FixedC2C<float, 10000, 1000, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 10, 10> c10000;  // 2554 -> 3956
-const FixedC2C<float, 10800, 720, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 15, 3> c10800;  // 2486 -> 3622
-const FixedC2C<float, 12288, 512, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 12> c12288;  // 2616 -> 3760
-const FixedC2C<float, 15360, 512, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 15> c15360;  // 2775 -> 3788
+const FixedC2C<float, 125, 9, 28, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 5> c125;  // 4060 -> 4314
+const FixedC2C<float, 270, 23, 11, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 3> c270;  // 4145 -> 4464
+const FixedC2C<float, 384, 16, 16, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 12> c384;  // 4657 -> 4974
+const FixedC2C<float, 540, 45, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 6> c540;  // 4116 -> 4361
+const FixedC2C<float, 800, 32, 8, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25> c800;  // 4328 -> 4811
+const FixedC2C<float, 1000, 100, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 10> c1000;  // 4081 -> 4604
+const FixedC2C<float, 1440, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 10> c1440;  // 3897 -> 4952
+const FixedC2C<float, 1600, 54, 4, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 10, 5> c1600;  // 4194 -> 4438
+const FixedC2C<float, 2025, 81, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 9, 9> c2025;  // 3247 -> 4518
+const FixedC2C<float, 2400, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 10> c2400;  // 4077 -> 4806
+const FixedC2C<float, 2700, 225, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 12> c2700;  // 4227 -> 4513
+const FixedC2C<float, 3125, 125, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 25, 5> c3125;  // 4028 -> 4646
+const FixedC2C<float, 3888, 432, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 6, 6> c3888;  // 2701 -> 4333
+const FixedC2C<float, 4374, 486, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 6> c4374;  // 2952 -> 4376
+const FixedC2C<float, 4860, 540, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9, 6> c4860;  // 2075 -> 3879
+const FixedC2C<float, 5625, 375, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 15, 15> c5625;  // 3456 -> 3886
+const FixedC2C<float, 6075, 405, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 3, 3, 3> c6075;  // 2612 -> 3175
+const FixedC2C<float, 6480, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 9, 9, 5> c6480;  // 2106 -> 4432
+const FixedC2C<float, 7290, 810, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9, 9> c7290;  // 2494 -> 4064
+const FixedC2C<float, 8100, 900, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 9, 9> c8100;  // 2428 -> 3909
+const FixedC2C<float, 9600, 640, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 8, 5> c9600;  // 2347 -> 3779
+const FixedC2C<float, 10368, 864, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 12, 6> c10368;  // 2554 -> 3779
+const FixedC2C<float, 12000, 480, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 15> c12000;  // 2418 -> 3702
+const FixedC2C<float, 14400, 960, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 15, 4> c14400;  // 2646 -> 3489
 }  // namespace
 }  // namespace sfft

@@ -1,5 +1,5 @@
 // fixed_single2.hip -- compile-time plans, complex FP32, single pass: part
-// 3 of 4 of the fast lengths 2^a 3^b 5^c in [270, 19200] that have no
+// 3 of 4 of the fast lengths 2^a 3^b 5^c in [64, 19200] that have no
 // hand-tuned kernel.  For each length two candidates were measured against the
 // runtime-planned pass (batched launches of about 256 MB, cold inputs): the
 // radices the runtime cost model picks at 16 elements per thread and at 32
@@ -7,8 +7,8 @@
 // about 256 threads per block.  A length is listed only where its faster
 // candidate beat the runtime pass by 5 % or more (the rate pair in GB/s, runtime
 // -> compiled, follows each line).  Lengths where the candidates lost (up to
-// 2.4x slower, mostly lengths above 10000 at 600 to 1000 threads per
-// transform) keep the runtime pass.
+// 2.4x slower: spilled registers at up to 1024 threads per transform) keep the
+// runtime pass.
 // FixedC2C<type, N, threads per transform, transforms per block,
 //          Sched<prefetch stage, min waves per SIMD, pair loads, prefetch first,
 //          LDS pad shift>, variant, radices...>
@@ -16,28 +16,29 @@
 
 namespace sfft {
 namespace {
-const FixedC2C<float, 300, 13, 19, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 12> c300;  // 4325 -> 4616
-const FixedC2C<float, 486, 54, 4, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 6> c486;  // 3892 -> 4497
-const FixedC2C<float, 648, 72, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 6> c648;  // 3740 -> 4407
-const FixedC2C<float, 864, 72, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 6> c864;  // 4476 -> 4764
-const FixedC2C<float, 1215, 135, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 9, 9> c1215;  // 3274 -> 4291
-const FixedC2C<float, 1500, 150, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 10, 10> c1500;  // 3823 -> 4446
-const FixedC2C<float, 1728, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 12> c1728;  // 4096 -> 4548
-const FixedC2C<float, 2187, 243, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 3> c2187;  // 3574 -> 4633
-const FixedC2C<float, 2560, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 10> c2560;  // 4306 -> 4838
-const FixedC2C<float, 2916, 324, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 6, 6> c2916;  // 2260 -> 3896
-const FixedC2C<float, 3456, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 12, 9> c3456;  // 2393 -> 4102
-const FixedC2C<float, 4050, 338, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 6, 3> c4050;  // 2410 -> 2933
-const FixedC2C<float, 4608, 192, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 12, 12> c4608;  // 2702 -> 4612
-const FixedC2C<float, 5184, 432, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 6, 6> c5184;  // 3114 -> 4454
-const FixedC2C<float, 5832, 648, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 9, 6> c5832;  // 2296 -> 3741
-const FixedC2C<float, 6250, 250, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 25, 10> c6250;  // 2927 -> 3734
-const FixedC2C<float, 6912, 576, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 12, 6, 6> c6912;  // 2369 -> 4112
-const FixedC2C<float, 7776, 864, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 9, 6> c7776;  // 2443 -> 4071
-const FixedC2C<float, 8748, 972, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 9, 9> c8748;  // 2518 -> 3752
-const FixedC2C<float, 10125, 675, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 15, 3> c10125;  // 2434 -> 2876
-const FixedC2C<float, 11520, 768, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 15, 3> c11520;  // 2508 -> 3740
-const FixedC2C<float, 12800, 512, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 16> c12800;  // 2398 -> 3399
-const FixedC2C<float, 17280, 576, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 6, 6> c17280;  // 1886 -> 2038
+const FixedC2C<float, 162, 18, 14, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 2> c162;  // 4076 -> 4777
+const FixedC2C<float, 288, 24, 10, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 2> c288;  // 4478 -> 4880
+const FixedC2C<float, 405, 45, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 5> c405;  // 4007 -> 4673
+const FixedC2C<float, 576, 48, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 6, 6> c576;  // 4426 -> 4888
+const FixedC2C<float, 810, 90, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9> c810;  // 4286 -> 4849
+const FixedC2C<float, 1080, 90, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 12, 6> c1080;  // 4323 -> 4626
+const FixedC2C<float, 1458, 162, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 2> c1458;  // 3096 -> 4339
+const FixedC2C<float, 1620, 180, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 12, 9> c1620;  // 3928 -> 4622
+const FixedC2C<float, 2160, 180, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 12, 12> c2160;  // 4366 -> 4645
+const FixedC2C<float, 2430, 270, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 9, 9, 3> c2430;  // 2060 -> 3701
+const FixedC2C<float, 2880, 96, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 6> c2880;  // 4489 -> 4919
+const FixedC2C<float, 3240, 270, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 6, 6> c3240;  // 2167 -> 3050
+const FixedC2C<float, 4000, 160, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 5> c4000;  // 2794 -> 4690
+const FixedC2C<float, 4500, 188, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 15, 12> c4500;  // 2461 -> 3238
+const FixedC2C<float, 5120, 171, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 5> c5120;  // 2876 -> 4420
+const FixedC2C<float, 5760, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 12> c5760;  // 3084 -> 4608
+const FixedC2C<float, 6144, 205, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 6> c6144;  // 3192 -> 4298
+const FixedC2C<float, 6561, 729, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 9> c6561;  // 2575 -> 4060
+const FixedC2C<float, 7680, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 16, 15> c7680;  // 3422 -> 4318
+const FixedC2C<float, 8640, 720, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 12, 5> c8640;  // 2257 -> 3813
+const FixedC2C<float, 10000, 1000, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 10, 10> c10000;  // 2554 -> 3956
+const FixedC2C<float, 10800, 720, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 15, 15, 3> c10800;  // 2486 -> 3622
+const FixedC2C<float, 12288, 512, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 12> c12288;  // 2616 -> 3760
+const FixedC2C<float, 15360, 512, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 32, 15> c15360;  // 2775 -> 3788
 }  // namespace
 }  // namespace sfft

@@ -1,5 +1,5 @@
 // fixed_single3.hip -- compile-time plans, complex FP32, single pass: part
-// 4 of 4 of the fast lengths 2^a 3^b 5^c in [270, 19200] that have no
+// 4 of 4 of the fast lengths 2^a 3^b 5^c in [64, 19200] that have no
 // hand-tuned kernel.  For each length two candidates were measured against the
 // runtime-planned pass (batched launches of about 256 MB, cold inputs): the
 // radices the runtime cost model picks at 16 elements per thread and at 32
@@ -7,8 +7,8 @@
 // about 256 threads per block.  A length is listed only where its faster
 // candidate beat the runtime pass by 5 % or more (the rate pair in GB/s, runtime
 // -> compiled, follows each line).  Lengths where the candidates lost (up to
-// 2.4x slower, mostly lengths above 10000 at 600 to 1000 threads per
-// transform) keep the runtime pass.
+// 2.4x slower: spilled registers at up to 1024 threads per transform) keep the
+// runtime pass.
 // FixedC2C<type, N, threads per transform, transforms per block,
 //          Sched<prefetch stage, min waves per SIMD, pair loads, prefetch first,
 //          LDS pad shift>, variant, radices...>
@@ -16,28 +16,29 @@
 
 namespace sfft {
 namespace {
-const FixedC2C<float, 324, 36, 7, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 6, 6> c324;  // 4004 -> 4571
-const FixedC2C<float, 500, 50, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 5> c500;  // 4545 -> 4950
-const FixedC2C<float, 729, 81, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9> c729;  // 4486 -> 4818
-const FixedC2C<float, 972, 108, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 9> c972;  // 4137 -> 4861
-const FixedC2C<float, 1296, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 9> c1296;  // 3627 -> 5000
-const FixedC2C<float, 1536, 128, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 6> c1536;  // 4514 -> 4749
-const FixedC2C<float, 1944, 216, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 6, 6, 6> c1944;  // 3116 -> 4305
-const FixedC2C<float, 2250, 225, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 10> c2250;  // 3852 -> 4323
-const FixedC2C<float, 2592, 96, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 9, 9> c2592;  // 3649 -> 4711
-const FixedC2C<float, 3072, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 12> c3072;  // 4616 -> 4856
-const FixedC2C<float, 3645, 405, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 5> c3645;  // 2636 -> 4313
-const FixedC2C<float, 4320, 160, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 9> c4320;  // 2651 -> 4137
-const FixedC2C<float, 4800, 160, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 10> c4800;  // 2699 -> 4347
-const FixedC2C<float, 5400, 360, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 8, 3> c5400;  // 2795 -> 3300
-const FixedC2C<float, 6000, 400, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 16, 15> c6000;  // 3631 -> 4166
-const FixedC2C<float, 6400, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 8> c6400;  // 3649 -> 4520
-const FixedC2C<float, 7200, 240, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 15> c7200;  // 3394 -> 4331
-const FixedC2C<float, 8000, 320, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 10> c8000;  // 2433 -> 3788
-const FixedC2C<float, 9216, 768, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 6, 6> c9216;  // 2393 -> 3904
-const FixedC2C<float, 10240, 683, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 8, 5> c10240;  // 2426 -> 3817
-const FixedC2C<float, 11664, 432, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 9, 9, 9> c11664;  // 1962 -> 3545
-const FixedC2C<float, 12960, 480, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 9, 9, 5> c12960;  // 1917 -> 3446
-const FixedC2C<float, 19200, 640, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 10, 10, 6> c19200;  // 1818 -> 2153
+const FixedC2C<float, 243, 27, 9, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 3> c243;  // 4215 -> 4826
+const FixedC2C<float, 300, 13, 19, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 12> c300;  // 4325 -> 4616
+const FixedC2C<float, 486, 54, 4, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 6> c486;  // 3892 -> 4497
+const FixedC2C<float, 648, 72, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 6> c648;  // 3740 -> 4407
+const FixedC2C<float, 864, 72, 3, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 6> c864;  // 4476 -> 4764
+const FixedC2C<float, 1215, 135, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 9, 9> c1215;  // 3274 -> 4291
+const FixedC2C<float, 1500, 150, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 10, 10> c1500;  // 3823 -> 4446
+const FixedC2C<float, 1728, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 12> c1728;  // 4096 -> 4548
+const FixedC2C<float, 2187, 243, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 9, 3> c2187;  // 3574 -> 4633
+const FixedC2C<float, 2560, 256, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 10> c2560;  // 4306 -> 4838
+const FixedC2C<float, 2916, 324, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 9, 9, 6, 6> c2916;  // 2260 -> 3896
+const FixedC2C<float, 3456, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 12, 9> c3456;  // 2393 -> 4102
+const FixedC2C<float, 4050, 338, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 6, 3> c4050;  // 2410 -> 2933
+const FixedC2C<float, 4608, 192, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 12, 12> c4608;  // 2702 -> 4612
+const FixedC2C<float, 5184, 432, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 6, 6> c5184;  // 3114 -> 4454
+const FixedC2C<float, 5832, 648, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 9, 6> c5832;  // 2296 -> 3741
+const FixedC2C<float, 6250, 250, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 25, 10> c6250;  // 2927 -> 3734
+const FixedC2C<float, 6912, 576, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 12, 6, 6> c6912;  // 2369 -> 4112
+const FixedC2C<float, 7776, 864, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 9, 6> c7776;  // 2443 -> 4071
+const FixedC2C<float, 8748, 972, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 9, 9, 9> c8748;  // 2518 -> 3752
+const FixedC2C<float, 10125, 675, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 15, 15, 3> c10125;  // 2434 -> 2876
+const FixedC2C<float, 11520, 768, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 16, 16, 15, 3> c11520;  // 2508 -> 3740
+const FixedC2C<float, 12800, 512, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25, 16> c12800;  // 2398 -> 3399
+const FixedC2C<float, 17280, 576, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 15, 6, 6> c17280;  // 1886 -> 2038
 }  // namespace
 }  // namespace sfft
