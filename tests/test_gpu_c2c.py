@@ -227,6 +227,31 @@ def test_two_pass_sizes(n, batch):
     assert po.rel_l2(z, po.ifft(y.astype(np.complex128))) <= TOL32
 
 
+@pytest.mark.parametrize("n,n1", [(16200, 0), (18432, 0), (16875, 0), (10935, 135), (13122, 81),
+                                  (15552, 192), (62208, 0), (104976, 0)])
+def test_planner_split_rules(n, n1):
+    """Lengths the planner splits by rule or by its measured split table:
+    lengths from 16000 without a compiled single pass, three shorter lengths
+    listed in the table (first-pass column length n1), and table lengths
+    above; the oracle checks forward and inverse."""
+    x = synth.complex_normal(n + 11, (3, n), np.complex64)
+    f = fft_amd.FFT(n)
+    inf = f.plan.info()
+    assert inf["passes"] == 2, inf
+    y = _run(n, x)
+    assert po.rel_l2(y, po.fft(x.astype(np.complex128))) <= TOL32
+    z = _run(n, y, inverse=True)
+    assert po.rel_l2(z, po.ifft(y.astype(np.complex128))) <= TOL32
+    if n1:
+        # the table's split: a plan asked for that split gives the same bits
+        p = fft_amd.Plan(n, L.C2C, L.F32, split=n1)
+        xd = _dev(x)
+        yd = torch.empty_like(xd)
+        p.execute_ptr(xd.data_ptr(), yd.data_ptr(), 3, 0, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(yd.cpu().numpy(), y)
+
+
 def test_two_pass_batch_matches_alone():
     """A two-pass batch (2^20 x 12): every transform equals the same
     transform run alone, bitwise, and the oracle."""

@@ -1,7 +1,15 @@
 """Every compiled schedule variant of every compile-time planned length (the
 autotuner may pick any of them): forward and inverse against numpy's FFT
 (exact to rounding in double, independent of this library), and variants
-with the same radices as the default give the default's bits."""
+with the same radices as the default give the default's bits.  The lengths
+of the measured kernel tables (fixed_single*, fixed_real_single*,
+fixed_f64_single*) are read from those sources, and each is also checked
+against the runtime-planned pass of the same length: bitwise where the
+radices match."""
+import glob
+import os
+import re
+
 import numpy as np
 import pytest
 import torch
@@ -22,8 +30,28 @@ CASES = [(1 << k, L.C2C, L.F32) for k in range(8, 15)] + [
     (n, L.R2C, L.F64) for n in (2048, 4096, 8192)]
 
 
+CSRC = os.path.join(os.path.dirname(__file__), "..", "fft_amd", "csrc")
+
+
+def _table(pattern, rx):
+    out = set()
+    for f in glob.glob(os.path.join(CSRC, pattern)):
+        out |= {int(m) for m in re.findall(rx, open(f).read())}
+    return sorted(out)
+
+
+TABLE_CASES = ([(n, L.C2C, L.F32) for n in _table("fixed_single*.hip", r"FixedC2C<float, (\d+),")] +
+               [(2 * h, L.R2C, L.F32)
+                for h in _table("fixed_real_single*.hip", r"FixedReal<float, (\d+),")] +
+               [(n, L.C2C, L.F64) for n in _table("fixed_f64_single*.hip", r"FixedC2C<double, (\d+),")])
+
+
 def _cases():
-    return CASES
+    return CASES + TABLE_CASES
+
+
+def test_tables_found():
+    assert len(TABLE_CASES) > 250, len(TABLE_CASES)
 
 
 @pytest.mark.parametrize("n,kind,prec", _cases())
@@ -63,3 +91,11 @@ def test_every_variant(n, kind, prec):
             assert err <= TOL[prec], (n, kind, prec, v, direction, p.info(), err)
             if p.info()["radices"] == default.info()["radices"]:
                 assert torch.equal(y, y0), (n, kind, prec, v, direction)
+        # the runtime-planned pass of the same length
+        rt = fft_amd.Plan(n, kind, prec, variant=-2)
+        y = torch.full(shape, float("nan"), dtype=odt, device="cuda")
+        rt.execute_ptr(x.data_ptr(), y.data_ptr(), batch, direction, sh)
+        torch.cuda.synchronize()
+        assert po.rel_l2(y.cpu().numpy(), want) <= TOL[prec], (n, kind, prec, direction)
+        if rt.info()["radices"] == default.info()["radices"]:
+            assert torch.equal(y, y0), (n, kind, prec, "runtime", direction)
