@@ -228,13 +228,18 @@ def fast_sizes_batched(torch, fft_amd, lo=64, hi=65536, target_bytes=256 << 20, 
         compiled += L.lib().sfft_plan_variants(n, L.C2C, L.F32) > 0
         for i in range(2):
             p.execute_ptr(bufs[i].data_ptr(), bufs[2 + i].data_ptr(), batch, 0, sh)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for r in range(2 * reps):
-            p.execute_ptr(bufs[r % 2].data_ptr(), bufs[2 + r % 2].data_ptr(), batch, 0, sh)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / (2 * reps)
+        # the faster of two timed rounds: a transient stall of the device in
+        # one round (seen once at 83 GB/s on a length that runs at 4 TB/s)
+        # does not stand for the length
+        ms = float("inf")
+        for _ in range(2):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for r in range(2 * reps):
+                p.execute_ptr(bufs[r % 2].data_ptr(), bufs[2 + r % 2].data_ptr(), batch, 0, sh)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = min(ms, e0.elapsed_time(e1) / (2 * reps))
         rates.append(16 * n * batch / ms / 1e6)
     del bufs
     torch.cuda.empty_cache()
