@@ -60,14 +60,24 @@ __device__ __forceinline__ void flip_im(C2<double> &x, uint32_t m) {
 //   PF_FIRST  in the prefetch stage, issue the prefetch before (1) or after
 //         (0) the next stage's twiddle loads
 //   PADSH one LDS pad element per 2^PADSH elements (kLdsNoPad: none)
+//   NT    non-temporal HBM accesses: bit 0 the loads of the input (a single
+//         pass, or a first column pass), bit 1 the stores of the final
+//         output (a single pass, or a second column pass), bit 2 also in the
+//         inverse real kernel of a real pair (else only its forward one);
+//         the intermediate of a two-pass plan always keeps the default policy.
+//         (On the real 2048 pair of the bench step, non-temporal accesses in
+//         the forward kernel made it 10 % faster and the inverse leg that
+//         follows it 10 % slower, with no net gain; they are not used there.)
 constexpr int kLdsNoPad = 30;
-template <int PFS_, int MINW_, int PAIR_ = 1, int PF_FIRST_ = 0, int PADSH_ = kLdsPadShift>
+template <int PFS_, int MINW_, int PAIR_ = 1, int PF_FIRST_ = 0, int PADSH_ = kLdsPadShift,
+          int NT_ = 0>
 struct Sched {
     static constexpr int PFS = PFS_;
     static constexpr int MINW = MINW_;
     static constexpr int PAIR = PAIR_;
     static constexpr int PF_FIRST = PF_FIRST_;
     static constexpr int PADSH = PADSH_;
+    static constexpr int NT = NT_;
 };
 
 // Element type of a compiled kernel: float / double (stored as is), or
@@ -107,8 +117,28 @@ struct Fixed {
     static constexpr uint32_t EB = sizeof(C2<T>);  // bytes per element in LDS
     static constexpr uint32_t DB =                  // and in HBM
         ST::native ? sizeof(C2<T>) : 2 * sizeof(uint16_t);
-    static __device__ __forceinline__ C2<T> ld(const char *p) { return ST::load(p, 0); }
-    static __device__ __forceinline__ void st(char *p, C2<T> v) { ST::store(p, 0, v); }
+    static constexpr bool NT_MODE = MODE != kModeC2R || (SCH::NT & 4);
+    static constexpr bool NT_LD = ST::native && (SCH::NT & 1) && NT_MODE && MODE != kModeColCol;
+    static constexpr bool NT_ST = ST::native && (SCH::NT & 2) && NT_MODE && MODE != kModeColRow;
+    using V2 = T __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ C2<T> ld(const char *p) {
+        if constexpr (NT_LD) {
+            const V2 v = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(p));
+            return {v.x, v.y};
+        } else {
+            return ST::load(p, 0);
+        }
+    }
+    static __device__ __forceinline__ void st(char *p, C2<T> v) {
+        if constexpr (NT_ST) {
+            V2 w;
+            w.x = v.x;
+            w.y = v.y;
+            __builtin_nontemporal_store(w, reinterpret_cast<V2 *>(p));
+        } else {
+            ST::store(p, 0, v);
+        }
+    }
     static constexpr size_t LDS_BYTES =
         (NST > 1 || MODE != kModeC2C) ? (size_t)STRIDE * EB * TPW : 0;
 

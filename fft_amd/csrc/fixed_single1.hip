@@ -7,8 +7,11 @@
 // about 256 threads per block.  A length is listed only where its faster
 // candidate beat the runtime pass by 5 % or more (the rate pair in GB/s, runtime
 // -> compiled, follows each line).  Lengths where the candidates lost (up to
-// 2.4x slower: spilled registers at up to 1024 threads per transform) keep the
-// runtime pass.
+// 2.4x slower, mostly lengths above 10000 at 600 to 1000 threads per
+// transform) keep the runtime pass.  Lines whose schedule ends in 3 use
+// non-temporal input loads and output stores (Sched NT): 800, 1536, 1728,
+// 2160, 2430, 2560, 3072, 7200, 7680, 12288, 12800, 17280 and 19200 measured
+// 7-18 % faster with them, others up to 2.5x slower.
 // FixedC2C<type, N, threads per transform, transforms per block,
 //          Sched<prefetch stage, min waves per SIMD, pair loads, prefetch first,
 //          LDS pad shift>, variant, radices...>
@@ -20,7 +23,7 @@ const FixedC2C<float, 125, 9, 28, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 25, 5> c12
 const FixedC2C<float, 270, 23, 11, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 3> c270;  // 4145 -> 4464
 const FixedC2C<float, 384, 16, 16, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 12> c384;  // 4657 -> 4974
 const FixedC2C<float, 540, 45, 5, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 15, 6, 6> c540;  // 4116 -> 4361
-const FixedC2C<float, 800, 32, 8, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 25> c800;  // 4328 -> 4811
+const FixedC2C<float, 800, 32, 8, Sched<0, 0, 0, 0, kLdsPadShift, 3>, 0, 32, 25> c800;  // 4328 -> 4811
 const FixedC2C<float, 1000, 100, 2, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 10, 10, 10> c1000;  // 4081 -> 4604
 const FixedC2C<float, 1440, 144, 1, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 12, 12, 10> c1440;  // 3897 -> 4952
 const FixedC2C<float, 1600, 54, 4, Sched<0, 0, 0, 0, kLdsPadShift>, 0, 32, 10, 5> c1600;  // 4194 -> 4438
