@@ -343,9 +343,9 @@ static bool layout_columns(int n, int prec, int max_cols, const PlanHints *h, Pa
 static void use_fixed_columns(PassPlan &pp, size_t n, int prec, int mode) {
     const FixedKernel *fk = find_fixed((int)n, prec, 2, -1);
     if (!fk) return;
-    // FP64 compiled columns measured faster only with 512 or more columns
-    // per transform (2^18 = 512 x 512: 1.81 -> 2.26 TB/s; 2^16 = 128 x 512:
-    // 2.47 -> 2.13)
+    // FP64 compiled columns measured faster than the runtime-planned pass
+    // only with 512 or more columns per transform (2^18 = 512 x 512: 1.81
+    // -> 2.26 TB/s; 2^16 = 128 x 512: 2.47 runtime-planned, 2.13 compiled)
     if (arith_f64(prec) && pp.slots_per_transform < 512) return;
     std::vector<int> rs(fk->radices, fk->radices + fk->nst);
     int ept = 0;
