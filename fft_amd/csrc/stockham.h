@@ -289,13 +289,32 @@ struct Block {
             // 32-bit add and one 64-bit shift-add per element address
             const char *sp = slot_ptr(in, active ? ibase(g) : 0);
             const uint32_t es = (uint32_t)iin_.elem_stride, rstep = (uint32_t)nb * es;
+            bool nt = false;
+            if constexpr (St::native) nt = post_tw_ != 0;
+            if (nt) {
+                // the first pass of a two-pass plan reads its input once:
+                // non-temporal loads (the second pass reads the intermediate
+                // with the default policy, from the caches where it can)
 #pragma unroll
-            for (int b = 0; b < B; ++b) {
-                const uint32_t j = (uint32_t)(lid + b * tpt_);
-                if (ok(b)) {
+                for (int b = 0; b < B; ++b) {
+                    const uint32_t j = (uint32_t)(lid + b * tpt_);
+                    if (ok(b)) {
 #pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        v[b][r] = St::load(sp, (int64_t)(j * es + (uint32_t)r * rstep));
+                        for (int r = 0; r < R; ++r) {
+                            if constexpr (St::native)
+                                v[b][r] = St::load_nt(sp, (int64_t)(j * es + (uint32_t)r * rstep));
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < B; ++b) {
+                    const uint32_t j = (uint32_t)(lid + b * tpt_);
+                    if (ok(b)) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            v[b][r] = St::load(sp, (int64_t)(j * es + (uint32_t)r * rstep));
+                    }
                 }
             }
             if (conj_in_) {

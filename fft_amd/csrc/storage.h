@@ -61,6 +61,12 @@ struct Storage<T, StoreNative> {
     __device__ static C2<T> load(const void *p, int64_t i) {
         return static_cast<const C2<T> *>(p)[i];
     }
+    // a load marked non-temporal (data read once)
+    __device__ static C2<T> load_nt(const void *p, int64_t i) {
+        using V2 = T __attribute__((ext_vector_type(2)));
+        const V2 v = __builtin_nontemporal_load(static_cast<const V2 *>(p) + i);
+        return {v.x, v.y};
+    }
     __device__ static void store(void *p, int64_t i, C2<T> v) { static_cast<C2<T> *>(p)[i] = v; }
 };
 
