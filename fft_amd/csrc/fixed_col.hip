@@ -21,10 +21,11 @@ const FixedCol<float, 256, 16, 32, Sched<0, 0, 0, 0, kLdsPadShift, 1>, 0, 16, 16
 const FixedCol<float, 512, 32, 16, Sched<0, 0, 0, 0, kLdsPadShift, 1>, 0, 16, 16, 2> col512;
 const FixedCol<float, 1024, 64, 8, Sched<0, 0, 0, 0>, 0, 16, 16, 4> col1024;
 const FixedCol<float, 2048, 128, 8, Sched<0, 0, 0, 0>, 0, 16, 16, 8> col2048;
-// complex FP64 (16-byte values: half the columns per LDS budget); length 256
-// keeps the runtime-planned pass (compiled: 2^16 x 256 2.47 -> 2.10 TB/s,
-// 2^17 x 128 2.22 -> 2.04)
-const FixedCol<double, 512, 32, 8, Sched<0, 0, 0, 0>, 0, 16, 16, 2> col512d;
-const FixedCol<double, 1024, 64, 4, Sched<0, 0, 0, 0>, 0, 16, 16, 4> col1024d;
+// complex FP64 (16-byte values: half the columns per LDS budget), first-pass
+// input loads non-temporal (2^20 x 16 1.75 -> 2.02 TB/s, 2^18 x 64 2.22 ->
+// 2.50); length 256 keeps the runtime-planned pass (compiled: 2^16 x 256
+// 2.47 -> 2.10 TB/s, 2^17 x 128 2.22 -> 2.04)
+const FixedCol<double, 512, 32, 8, Sched<0, 0, 0, 0, kLdsPadShift, 1>, 0, 16, 16, 2> col512d;
+const FixedCol<double, 1024, 64, 4, Sched<0, 0, 0, 0, kLdsPadShift, 1>, 0, 16, 16, 4> col1024d;
 }  // namespace
 }  // namespace sfft

@@ -289,12 +289,15 @@ struct Block {
             // 32-bit add and one 64-bit shift-add per element address
             const char *sp = slot_ptr(in, active ? ibase(g) : 0);
             const uint32_t es = (uint32_t)iin_.elem_stride, rstep = (uint32_t)nb * es;
+            // the first pass of a two-pass plan reads its input once:
+            // non-temporal loads (the second pass reads the intermediate with
+            // the default policy, from the caches where it can).  Not for
+            // tiles narrower than a 128-byte line: neighbouring tiles share
+            // those lines, and non-temporal loads fetched them again (2^24
+            // F32, 4-column tiles: 1.42 -> 1.21 TB/s).
             bool nt = false;
-            if constexpr (St::native) nt = post_tw_ != 0;
+            if constexpr (St::native) nt = post_tw_ != 0 && tpw_ * St::kBytes >= 128;
             if (nt) {
-                // the first pass of a two-pass plan reads its input once:
-                // non-temporal loads (the second pass reads the intermediate
-                // with the default policy, from the caches where it can)
 #pragma unroll
                 for (int b = 0; b < B; ++b) {
                     const uint32_t j = (uint32_t)(lid + b * tpt_);
