@@ -59,8 +59,8 @@ def test_size_helpers_reference_properties():
 
 
 def test_size_helper_quirks():
-    # Values the survey recorded from the reference (a1, a11): never a
-    # multiple of 5; RealFFT helpers built on the half size.
+    # Values the reference's size helpers return (signalsmith-fft.h:317-348,
+    # :403-441): never a multiple of 5; RealFFT helpers built on the half size.
     assert po.size_minimum(5) == 6 and po.size_minimum(10) == 12
     assert po.size_minimum(1000) == 1024
     assert po.size_minimum(3000) == 3072 and po.size_maximum(3000) == 2304
