@@ -173,7 +173,9 @@ constexpr size_t kTwoPassFrom = 16000;
 // -> 2.17 TB/s).  The first three lengths, below 16000, have no compiled
 // single pass and measured faster split than in one runtime-planned pass
 // (10935: 1.81 -> 2.14 TB/s, 13122: 1.61 -> 2.06, 15552: 1.79 -> 2.29).
-// Sorted by length.
+// The last two, 2^21 and 2^22, measured 3-5 % faster with first-pass
+// columns of 4096 in two runs each (2^21 x 8: 1.69-1.74 -> 1.79-1.81;
+// 2^22 x 4: 1.64-1.70 -> 1.74).  Sorted by length.
 struct MeasuredSplit {
     uint32_t n, n1;
 };
@@ -193,6 +195,7 @@ static const MeasuredSplit kMeasuredSplits[] = {
     {163840, 160}, {164025, 729}, {174960, 432}, {182250, 375}, {184320, 1024}, {196608, 1024},
     {202500, 500}, {204800, 1024}, {216000, 432}, {218700, 729}, {225000, 360}, {243000, 500},
     {245760, 1024}, {248832, 512}, {253125, 625},
+    {2097152, 4096}, {4194304, 4096},
 };
 
 // The same for real FP32 two-pass plans (RealFFT, not the half-bin shifted
