@@ -224,11 +224,22 @@ static const MeasuredSplit kMeasuredRealSplits[] = {
     {245760, 1024}, {248832, 512}, {253125, 625}, {256000, 512}, {262144, 1024},
 };
 
-// Complex plans with 16-bit storage or FP32 storage under FP64 arithmetic
-// (F32X) at 2^20: first-pass columns of 4096 measured faster than the scored
-// 1024 (f16 110-113 -> 105-106 us, F32X 222-225 -> 196 us at 2^20 x 16).
-static const MeasuredSplit kMeasuredSplitsStored[] = {
-    {1048576, 4096},
+// The same for the power-of-two two-pass lengths of the other precisions
+// (complex, about 128 MB per launch, every split timed twice; listed where
+// both timings beat both of the scored split's by more than 5 %, and at 2^20
+// for 16-bit storage and F32X, where 2^20 x 16 measured f16 110-113 -> 105-106
+// us and F32X 222-225 -> 196 us).  16-bit storage (IEEE half and bfloat16:
+// 2^18 1.16-1.22x, 2^19 1.11-1.18x, 2^21 1.07-1.08x):
+static const MeasuredSplit kMeasuredSplits16[] = {
+    {262144, 2048}, {524288, 2048}, {1048576, 4096}, {2097152, 2048},
+};
+// FP32 storage under FP64 arithmetic (F32X; 2^19 1.13x):
+static const MeasuredSplit kMeasuredSplitsF32X[] = {
+    {524288, 2048}, {1048576, 4096},
+};
+// FP64 (2^21 1.17x, 2^22 1.10x):
+static const MeasuredSplit kMeasuredSplitsF64[] = {
+    {2097152, 4096}, {4194304, 4096},
 };
 
 template <size_t K>
@@ -508,8 +519,11 @@ Plan::Plan(size_t n, int kind, int prec, const PlanHints *h) : n_(n), kind_(kind
             table_n1 = kind == kC2C   ? lookup_split(kMeasuredSplits, cn_)
                        : kind == kR2C ? lookup_split(kMeasuredRealSplits, cn_)
                                       : 0;
-        else if (free_split && kind == kC2C && (stored_16(prec) || prec == kF32X))
-            table_n1 = lookup_split(kMeasuredSplitsStored, cn_);
+        else if (free_split && kind == kC2C)
+            table_n1 = stored_16(prec)  ? lookup_split(kMeasuredSplits16, cn_)
+                       : prec == kF32X ? lookup_split(kMeasuredSplitsF32X, cn_)
+                       : prec == kF64  ? lookup_split(kMeasuredSplitsF64, cn_)
+                                       : 0;
         search(table_n1);
         if (!best_n1 && table_n1) search(0);
         if (!best_n1 && !ok)
