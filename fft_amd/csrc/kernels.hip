@@ -58,9 +58,12 @@ template <typename T>
 hipError_t launch_real_post(C2<T> *out, int64_t dist, int h, size_t batch, const C2<T> *tmi,
                             int half_shift, hipStream_t stream) {
     if (batch == 0) return hipSuccess;
+    // about 2048 workgroups in all (at least 64 per transform), so that a
+    // small batch of long transforms still covers every CU
     const int pairs = h / 2 + 1;
+    const size_t want = batch >= 32 ? 64 : 2048 / batch;
     unsigned gx = (unsigned)((pairs + 255) / 256);
-    if (gx > 64) gx = 64;
+    if (gx > want) gx = (unsigned)want;
     const unsigned gy = (unsigned)(batch < 65535 ? batch : 65535);
     hipLaunchKernelGGL((k_real_post<T>), dim3(gx, gy), dim3(256), 0, stream, out, dist, h,
                        (int)batch, tmi, half_shift);

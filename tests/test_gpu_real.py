@@ -133,6 +133,21 @@ def test_real_float_sizes(n, mod):
     assert po.rel_l2(z, po.real_ifft(y.astype(np.complex128), half_shift=mod)) <= TOL32
 
 
+@pytest.mark.parametrize("n,batch", [(1 << 24, 1), (1 << 22, 2)])
+@pytest.mark.parametrize("mod", [False, True])
+def test_real_long_few_frames(n, batch, mod):
+    """Long two-pass real transforms in small batches (the forward pass'
+    pairwise post-process then spreads one transform over many workgroups),
+    forward and inverse against the oracle (signalsmith-fft.h:446-502)."""
+    x = synth.real_uniform(n + 5, (batch, n), np.float32)
+    y = _fwd(n, x, mod, dtype=torch.float32)
+    err = po.rel_l2(y, po.real_fft(x.astype(np.float64), half_shift=mod))
+    print("R2C", n, batch, mod, "forward rel L2", err)
+    assert err <= TOL32
+    z = _inv(n, y, mod, dtype=torch.float32)
+    assert po.rel_l2(z, po.real_ifft(y.astype(np.complex128), half_shift=mod)) <= TOL32
+
+
 def test_real_setters_mirror_reference():
     t = np.load(os.path.join(GOLDEN, "sizes.npz"))
     rf = fft_amd.RealFFT(2)
