@@ -37,17 +37,22 @@ def _check_rows(run, x, y, per_in, per_out):
     return sorted(rows)
 
 
-@pytest.mark.parametrize("n,batch", [(16384, 40000), (1 << 20, 600)],
-                         ids=["single-pass-16384", "two-pass-2^20"])
-def test_c2c_batch_over_4gib(n, batch):
-    f = fft_amd.FFT(n)
-    x = torch.empty(batch, n, dtype=torch.complex64, device="cuda")
-    assert x.numel() * 8 > GIB4
+@pytest.mark.parametrize("n,batch,dtype", [(16384, 40000, torch.complex64),
+                                           (1 << 20, 600, torch.complex64),
+                                           (4096, 70000, torch.complex128),
+                                           (1 << 18, 1100, torch.complex128)],
+                         ids=["single-pass-16384", "two-pass-2^20", "f64-single-pass-4096",
+                              "f64-two-pass-2^18"])
+def test_c2c_batch_over_4gib(n, batch, dtype):
+    f = fft_amd.FFT(n, dtype=dtype)
+    eb = 16 if dtype == torch.complex128 else 8
+    x = torch.empty(batch, n, dtype=dtype, device="cuda")
+    assert x.numel() * eb > GIB4
     fft_amd.fill(x, seed=n, dist="normal")
     y = torch.empty_like(x)
     f.fft(x, y)
     torch.cuda.synchronize()
-    rows = _check_rows(f.fft, x, y, 8 * n, 8 * n)
+    rows = _check_rows(f.fft, x, y, eb * n, eb * n)
     print("C2C", n, batch, "passes", f.plan.info()["passes"], "rows checked", rows)
     del x, y
     torch.cuda.empty_cache()
