@@ -47,7 +47,9 @@ const FixedCol<float, 540, 45, 16, Sched<0, 0, 0, 0, kLdsNoPad, 1>, 0, 15, 6, 6>
 // 4-10 % faster than nine threads per column at 32 columns (16875: 2.04 ->
 // 2.23 TB/s, 31250: 2.03 -> 2.21), same bits.  The same change for other
 // lengths with idle threads in a stage measured slower (150 at five threads
-// and 64 columns, 135 at nine, 250 at ten, 324 at 18: medians -2 to -9 %).
+// and 64 columns, 135 at nine, 250 at ten, 324 at 18: medians -2 to -9 %),
+// and 64 columns a block for 81, 135, 144, 150 and 160 was neutral (medians
+// 0.98-1.01).
 const FixedCol<float, 125, 5, 64, Sched<0, 0, 0, 0, kLdsNoPad, 1>, 0, 25, 5> col125;
 const FixedCol<float, 729, 81, 8, Sched<0, 0, 0, 0, kLdsNoPad, 1>, 0, 9, 9, 9> col729;
 const FixedCol<float, 144, 12, 32, Sched<0, 0, 0, 0, kLdsNoPad, 1>, 0, 12, 12> col144;
