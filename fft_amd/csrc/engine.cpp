@@ -170,9 +170,11 @@ constexpr size_t kTwoPassFrom = 16000;
 // have compiled kernels was timed (batched, 256 MB launches), and a length
 // is listed where its fastest split beat the planner's scored choice by
 // more than 5 % (81 of 185 lengths; up to 1.22x, median of all lengths 2.04
-// -> 2.17 TB/s).  The first three lengths, below 16000, have no compiled
+// -> 2.17 TB/s).  The first five lengths, below 16000, have no compiled
 // single pass and measured faster split than in one runtime-planned pass
-// (10935: 1.81 -> 2.14 TB/s, 13122: 1.61 -> 2.06, 15552: 1.79 -> 2.29).
+// (10935: 1.81 -> 2.14 TB/s, 12150: 2.25 -> 2.39, 13122: 1.61 -> 2.06,
+// 14580: 2.08 -> 2.37, 15552: 1.79 -> 2.29); 16000 measured 2.37 -> 2.62
+// TB/s with a first pass of 128 after the 125-length column kernel's change.
 // The last two, 2^21 and 2^22, measured 3-5 % faster with first-pass
 // columns of 4096 in two runs each (2^21 x 8: 1.69-1.74 -> 1.79-1.81;
 // 2^22 x 4: 1.64-1.70 -> 1.74).  Sorted by length.
@@ -180,7 +182,7 @@ struct MeasuredSplit {
     uint32_t n, n1;
 };
 static const MeasuredSplit kMeasuredSplits[] = {
-    {10935, 135}, {13122, 81}, {15552, 192},
+    {10935, 135}, {12150, 81}, {13122, 81}, {14580, 81}, {15552, 192}, {16000, 128},
     {18225, 81}, {19683, 243}, {24000, 160}, {34560, 192}, {34992, 144}, {36000, 160},
     {38880, 160}, {39366, 243}, {43200, 192}, {46656, 192}, {46875, 375}, {51200, 256},
     {54675, 243}, {56250, 250}, {60000, 375}, {64000, 160}, {64800, 160}, {67500, 300},
