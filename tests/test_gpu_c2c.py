@@ -229,6 +229,7 @@ def test_two_pass_sizes(n, batch):
 
 @pytest.mark.parametrize("n,n1", [(16200, 0), (18432, 0), (16875, 0), (10935, 135), (13122, 81),
                                   (15552, 192), (12150, 81), (14580, 81), (16000, 128),
+                                  (20000, 160), (60000, 150), (180000, 288),
                                   (62208, 0), (104976, 0), (1 << 21, 4096), (1 << 22, 4096)])
 def test_planner_split_rules(n, n1):
     """Lengths the planner splits by rule or by its measured split table:
