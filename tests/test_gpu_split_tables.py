@@ -27,7 +27,8 @@ ENGINE = os.path.join(os.path.dirname(__file__), "..", "fft_amd", "csrc", "engin
 
 def _split_table(name):
     src = open(ENGINE).read()
-    m = re.search(r"kMeasuredSplits%s\[\]\s*=\s*\{(.*?)\};" % name, src, re.S)
+    array = "kMeasuredRealSplits" if name == "Real" else "kMeasuredSplits" + name
+    m = re.search(r"%s\[\]\s*=\s*\{(.*?)\};" % array, src, re.S)
     assert m, name
     return [(int(a), int(b)) for a, b in re.findall(r"\{\s*(\d+)\s*,\s*(\d+)\s*\}", m.group(1))]
 
@@ -107,3 +108,38 @@ def test_measured_split(prec, n, n1):
     else:
         tol = 1e-7 if prec == L.F32X else 1e-12
         assert err <= tol and erri <= tol, (err, erri)
+
+
+def _real_cases():
+    t = _split_table("Real")
+    picks = {t[0], t[len(t) // 2], t[-1]} | {e for e in t if e[0] == 30000}
+    return sorted(picks)
+
+
+@pytest.mark.parametrize("h,n1", _real_cases(), ids=lambda v: str(v))
+def test_measured_real_split(h, n1):
+    """RealFFT<float> two-pass plans take their half length's table split
+    (first, middle, last and one re-measured entry): forward and inverse
+    against the oracle (signalsmith-fft.h:446-502), forward bitwise against
+    the plan asked for that split."""
+    n = 2 * h
+    plan = fft_amd.Plan(n, L.R2C, L.F32)
+    info = plan.info()
+    assert info["passes"] == 2 and math.prod(info["radices"]) == n1, info
+    x = synth.real_uniform(n + 3, (2, n), np.float32)
+    xd = torch.from_numpy(x).cuda()
+    sh = torch.cuda.current_stream().cuda_stream
+    y1 = torch.empty(2, h, dtype=torch.complex64, device="cuda")
+    y2 = torch.empty_like(y1)
+    z = torch.empty_like(xd)
+    plan.execute_ptr(xd.data_ptr(), y1.data_ptr(), 2, L.FORWARD, sh)
+    fft_amd.Plan(n, L.R2C, L.F32, split=n1).execute_ptr(xd.data_ptr(), y2.data_ptr(), 2,
+                                                       L.FORWARD, sh)
+    plan.execute_ptr(y1.data_ptr(), z.data_ptr(), 2, L.INVERSE, sh)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    y = y1.cpu().numpy()
+    err = po.rel_l2(y, po.real_fft(x.astype(np.float64)))
+    erri = po.rel_l2(z.cpu().numpy(), po.real_ifft(y.astype(np.complex128)))
+    print("R2C", n, n1, "forward rel L2", err, "inverse", erri)
+    assert err <= 1e-6 and erri <= 1e-6, (err, erri)
