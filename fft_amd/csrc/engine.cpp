@@ -177,7 +177,9 @@ constexpr size_t kTwoPassFrom = 16000;
 // TB/s with a first pass of 128 after the 125-length column kernel's change,
 // and a re-sweep of the lengths divisible by 125 then set ten more entries
 // (16875, 18000, 18750, 20000, 48000, 50000, 60000, 90000, 93750, 180000:
-// 1.06-1.20x over the previous choice).
+// 1.06-1.20x over the previous choice); a final re-sweep of all of them with
+// the final kernels set ten more (31104, 32400, 36864, 46080, 48600, 54000,
+// 55296, 147456, 184320, 196608: 1.07-1.17x).
 // The last two, 2^21 and 2^22, measured 3-5 % faster with first-pass
 // columns of 4096 in two runs each (2^21 x 8: 1.69-1.74 -> 1.79-1.81;
 // 2^22 x 4: 1.64-1.70 -> 1.74).  Sorted by length.
@@ -187,20 +189,22 @@ struct MeasuredSplit {
 static const MeasuredSplit kMeasuredSplits[] = {
     {10935, 135}, {12150, 81}, {13122, 81}, {14580, 81}, {15552, 192}, {16000, 128},
     {16875, 135}, {18000, 144}, {18225, 81}, {18750, 150}, {19683, 243}, {20000, 160},
-    {24000, 160}, {34560, 192}, {34992, 144}, {36000, 160}, {38880, 160}, {39366, 243},
-    {43200, 192}, {46656, 192}, {46875, 375}, {48000, 160}, {50000, 250}, {51200, 256},
-    {54675, 243}, {56250, 250}, {60000, 150}, {64000, 160}, {64800, 160}, {67500, 300},
-    {69120, 160}, {72000, 144}, {72900, 150}, {73728, 512}, {77760, 160}, {78125, 625},
-    {80000, 160}, {81000, 360}, {81920, 512}, {82944, 192}, {84375, 375}, {86400, 360},
-    {87480, 180}, {90000, 144}, {91125, 405}, {92160, 360}, {93312, 192}, {93750, 150},
-    {96000, 400}, {97200, 432}, {98304, 512}, {100000, 500}, {101250, 450}, {102400, 256},
-    {103680, 432}, {104976, 432}, {108000, 432}, {109350, 729}, {110592, 256}, {112500, 500},
-    {115200, 480}, {116640, 288}, {120000, 500}, {121500, 540}, {122880, 512}, {124416, 288},
-    {125000, 625}, {129600, 432}, {131220, 729}, {135000, 625}, {139968, 288}, {140625, 625},
-    {144000, 288}, {145800, 729}, {147456, 1024}, {155520, 360}, {156250, 625}, {157464, 729},
-    {163840, 160}, {164025, 729}, {174960, 432}, {180000, 288}, {182250, 375}, {184320, 1024},
-    {196608, 1024}, {202500, 500}, {204800, 1024}, {216000, 432}, {218700, 729}, {225000, 360},
-    {243000, 500}, {245760, 1024}, {248832, 512}, {253125, 625}, {2097152, 4096}, {4194304, 4096},
+    {24000, 160}, {31104, 128}, {32400, 144}, {34560, 192}, {34992, 144}, {36000, 160},
+    {36864, 144}, {38880, 160}, {39366, 243}, {43200, 192}, {46080, 256}, {46656, 192},
+    {46875, 375}, {48000, 160}, {48600, 243}, {50000, 250}, {51200, 256}, {54000, 240},
+    {54675, 243}, {55296, 128}, {56250, 250}, {60000, 150}, {64000, 160}, {64800, 160},
+    {67500, 300}, {69120, 160}, {72000, 144}, {72900, 150}, {73728, 512}, {77760, 160},
+    {78125, 625}, {80000, 160}, {81000, 360}, {81920, 512}, {82944, 192}, {84375, 375},
+    {86400, 360}, {87480, 180}, {90000, 144}, {91125, 405}, {92160, 360}, {93312, 192},
+    {93750, 150}, {96000, 400}, {97200, 432}, {98304, 512}, {100000, 500}, {101250, 450},
+    {102400, 256}, {103680, 432}, {104976, 432}, {108000, 432}, {109350, 729}, {110592, 256},
+    {112500, 500}, {115200, 480}, {116640, 288}, {120000, 500}, {121500, 540}, {122880, 512},
+    {124416, 288}, {125000, 625}, {129600, 432}, {131220, 729}, {135000, 625}, {139968, 288},
+    {140625, 625}, {144000, 288}, {145800, 729}, {147456, 144}, {155520, 360}, {156250, 625},
+    {157464, 729}, {163840, 160}, {164025, 729}, {174960, 432}, {180000, 288}, {182250, 375},
+    {184320, 360}, {196608, 192}, {202500, 500}, {204800, 1024}, {216000, 432}, {218700, 729},
+    {225000, 360}, {243000, 500}, {245760, 1024}, {248832, 512}, {253125, 625}, {2097152, 4096},
+    {4194304, 4096},
 };
 
 // The same for real FP32 two-pass plans (RealFFT, not the half-bin shifted
