@@ -213,26 +213,28 @@ static const MeasuredSplit kMeasuredSplits[] = {
 // together, and 107 of 177 half lengths keep a split that beat the scored
 // choice by more than 5 % (median of all 1.08x, up to 1.32x).  A re-sweep of
 // the half lengths divisible by 125 after the 125-length column change
-// added 30000 (1.07x).
+// added 30000 (1.07x), and a final re-sweep of all 177 set 23040 and 186624
+// (1.07x, 1.09x).
 static const MeasuredSplit kMeasuredRealSplits[] = {
-    {19440, 240}, {19683, 243}, {20000, 160}, {24000, 160}, {25000, 200}, {25920, 192},
-    {27648, 192}, {28125, 225}, {29160, 360}, {30000, 240}, {30720, 128}, {31104, 128},
-    {32805, 243}, {34560, 192}, {34992, 144}, {36000, 240}, {36450, 243}, {36864, 256},
-    {38880, 160}, {39366, 243}, {40000, 160}, {40960, 256}, {43200, 192}, {45000, 225},
-    {46080, 256}, {46656, 192}, {46875, 375}, {48000, 240}, {48600, 243}, {51200, 256},
-    {51840, 192}, {55296, 256}, {61440, 256}, {62208, 256}, {62500, 500}, {64000, 256},
-    {67500, 450}, {69984, 144}, {72000, 480}, {72900, 180}, {75000, 500}, {77760, 192},
-    {78125, 625}, {80000, 160}, {81000, 450}, {82944, 256}, {86400, 480}, {87480, 180},
-    {90000, 450}, {91125, 225}, {92160, 512}, {93312, 192}, {93750, 625}, {96000, 240},
-    {97200, 240}, {100000, 160}, {101250, 450}, {102400, 256}, {103680, 256}, {104976, 144},
-    {108000, 480}, {109350, 225}, {110592, 256}, {112500, 500}, {115200, 512}, {116640, 480},
-    {120000, 240}, {121500, 225}, {122880, 512}, {124416, 256}, {125000, 625}, {128000, 256},
-    {129600, 240}, {131220, 180}, {135000, 450}, {138240, 256}, {139968, 192}, {140625, 625},
-    {144000, 480}, {147456, 1024}, {150000, 240}, {151875, 625}, {153600, 1024}, {155520, 480},
-    {156250, 625}, {157464, 729}, {160000, 256}, {163840, 1024}, {165888, 512}, {168750, 625},
-    {172800, 480}, {174960, 240}, {180000, 450}, {182250, 450}, {184320, 1024}, {187500, 625},
-    {192000, 480}, {194400, 480}, {196608, 1024}, {204800, 1024}, {207360, 480}, {221184, 512},
-    {234375, 625}, {245760, 1024}, {248832, 512}, {253125, 625}, {256000, 512}, {262144, 1024},
+    {19440, 240}, {19683, 243}, {20000, 160}, {23040, 128}, {24000, 160}, {25000, 200},
+    {25920, 192}, {27648, 192}, {28125, 225}, {29160, 360}, {30000, 240}, {30720, 128},
+    {31104, 128}, {32805, 243}, {34560, 192}, {34992, 144}, {36000, 240}, {36450, 243},
+    {36864, 256}, {38880, 160}, {39366, 243}, {40000, 160}, {40960, 256}, {43200, 192},
+    {45000, 225}, {46080, 256}, {46656, 192}, {46875, 375}, {48000, 240}, {48600, 243},
+    {51200, 256}, {51840, 192}, {55296, 256}, {61440, 256}, {62208, 256}, {62500, 500},
+    {64000, 256}, {67500, 450}, {69984, 144}, {72000, 480}, {72900, 180}, {75000, 500},
+    {77760, 192}, {78125, 625}, {80000, 160}, {81000, 450}, {82944, 256}, {86400, 480},
+    {87480, 180}, {90000, 450}, {91125, 225}, {92160, 512}, {93312, 192}, {93750, 625},
+    {96000, 240}, {97200, 240}, {100000, 160}, {101250, 450}, {102400, 256}, {103680, 256},
+    {104976, 144}, {108000, 480}, {109350, 225}, {110592, 256}, {112500, 500}, {115200, 512},
+    {116640, 480}, {120000, 240}, {121500, 225}, {122880, 512}, {124416, 256}, {125000, 625},
+    {128000, 256}, {129600, 240}, {131220, 180}, {135000, 450}, {138240, 256}, {139968, 192},
+    {140625, 625}, {144000, 480}, {147456, 1024}, {150000, 240}, {151875, 625}, {153600, 1024},
+    {155520, 480}, {156250, 625}, {157464, 729}, {160000, 256}, {163840, 1024}, {165888, 512},
+    {168750, 625}, {172800, 480}, {174960, 240}, {180000, 450}, {182250, 450}, {184320, 1024},
+    {186624, 256}, {187500, 625}, {192000, 480}, {194400, 480}, {196608, 1024}, {204800, 1024},
+    {207360, 480}, {221184, 512}, {234375, 625}, {245760, 1024}, {248832, 512}, {253125, 625},
+    {256000, 512}, {262144, 1024},
 };
 
 // The same for the power-of-two two-pass lengths of the other precisions
